@@ -1,0 +1,444 @@
+// capi.cpp -- extern "C" boundary of libhmx (declared in include/hmx.h).
+// Every entry point converts C++ exceptions into an hmx_status code and a
+// thread-local message for hmx_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+
+#include "hmx_host.hpp"
+
+struct hmx_tree {
+    std::unique_ptr<hmx::ClusterTree> t;
+};
+struct hmx_blocktree {
+    std::unique_ptr<hmx::BlockTree> b;
+};
+struct hmx_lowrank {
+    hmx::LowRank lr;
+};
+struct hmx_hmatrix {
+    std::unique_ptr<hmx::HMatrix> h;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return HMX_OK;
+    } catch (const hmx::Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return HMX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return HMX_ERR_ARG;
+    } catch (...) {
+        g_err = "unknown error";
+        return HMX_ERR_ARG;
+    }
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) throw hmx::Error(HMX_ERR_ARG, what);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hmx_last_error(void) { return g_err.c_str(); }
+
+int hmx_version(int32_t* major, int32_t* minor, int32_t* patch) {
+    return guarded([&] {
+        if (major) *major = 0;
+        if (minor) *minor = 1;
+        if (patch) *patch = 0;
+    });
+}
+
+// ---- cluster tree ---------------------------------------------------------------
+int hmx_tree_build(const double* xyz, int64_t npoints, int32_t leaf, int32_t dims, hmx_tree** out) {
+    return guarded([&] {
+        need(out != nullptr, "hmx_tree_build: out is NULL");
+        *out = nullptr;
+        auto t = hmx::build_cluster_tree(xyz, npoints, leaf, dims);
+        *out = new hmx_tree{std::move(t)};
+    });
+}
+
+int hmx_tree_sizes(const hmx_tree* t, int64_t* nindex, int64_t* nnodes) {
+    return guarded([&] {
+        need(t && t->t, "hmx_tree_sizes: NULL tree");
+        if (nindex) *nindex = static_cast<int64_t>(t->t->index_map.size());
+        if (nnodes) *nnodes = static_cast<int64_t>(t->t->nodes.size());
+    });
+}
+
+int hmx_tree_export(const hmx_tree* t, int64_t* index_map, int64_t* start_idx, int64_t* end_idx,
+                    double* center, double* radius, int64_t* left, int64_t* right,
+                    int8_t* is_leaf) {
+    return guarded([&] {
+        need(t && t->t, "hmx_tree_export: NULL tree");
+        const hmx::ClusterTree& T = *t->t;
+        if (index_map) std::memcpy(index_map, T.index_map.data(), T.index_map.size() * sizeof(int64_t));
+        for (size_t k = 0; k < T.nodes.size(); k++) {
+            const hmx::ClusterNode& n = T.nodes[k];
+            if (start_idx) start_idx[k] = n.start;
+            if (end_idx) end_idx[k] = n.end;
+            if (center) std::memcpy(center + 3 * k, n.center, 3 * sizeof(double));
+            if (radius) radius[k] = n.radius;
+            if (left) left[k] = n.left;
+            if (right) right[k] = n.right;
+            if (is_leaf) is_leaf[k] = n.leaf ? 1 : 0;
+        }
+    });
+}
+
+int hmx_tree_info(const hmx_tree* t, int64_t* depth, int64_t* max_points, int64_t* min_points) {
+    return guarded([&] {
+        need(t && t->t && depth && max_points && min_points, "hmx_tree_info: NULL argument");
+        hmx::tree_info(*t->t, depth, max_points, min_points);
+    });
+}
+
+int hmx_tree_destroy(hmx_tree* t) {
+    delete t;
+    return HMX_OK;
+}
+
+// ---- block tree -------------------------------------------------------------------
+int hmx_blocktree_build(const hmx_tree* target, const hmx_tree* source, double eta,
+                        hmx_blocktree** out) {
+    return guarded([&] {
+        need(target && source && target->t && source->t && out, "hmx_blocktree_build: NULL argument");
+        *out = nullptr;
+        *out = new hmx_blocktree{hmx::build_block_tree(*target->t, *source->t, eta)};
+    });
+}
+
+int hmx_blocktree_merge_dense(hmx_blocktree* b) {
+    return guarded([&] {
+        need(b && b->b, "hmx_blocktree_merge_dense: NULL block tree");
+        hmx::merge_dense(*b->b);
+    });
+}
+
+int hmx_blocktree_sizes(const hmx_blocktree* b, int64_t* nnodes, int64_t* ndirect, int64_t* napprox) {
+    return guarded([&] {
+        need(b && b->b, "hmx_blocktree_sizes: NULL block tree");
+        std::vector<std::pair<int64_t, int64_t>> d, a;
+        hmx::traverse(*b->b, &d, &a);
+        if (nnodes) *nnodes = static_cast<int64_t>(b->b->nodes.size());
+        if (ndirect) *ndirect = static_cast<int64_t>(d.size());
+        if (napprox) *napprox = static_cast<int64_t>(a.size());
+    });
+}
+
+int hmx_blocktree_export(const hmx_blocktree* b, int64_t* target_node, int64_t* source_node,
+                         int8_t* is_admissible, int8_t* is_dense, int64_t* left, int64_t* right) {
+    return guarded([&] {
+        need(b && b->b, "hmx_blocktree_export: NULL block tree");
+        const auto& N = b->b->nodes;
+        for (size_t k = 0; k < N.size(); k++) {
+            if (target_node) target_node[k] = N[k].t;
+            if (source_node) source_node[k] = N[k].s;
+            if (is_admissible) is_admissible[k] = N[k].adm ? 1 : 0;
+            if (is_dense) is_dense[k] = N[k].dense ? 1 : 0;
+            if (left) left[k] = N[k].left;
+            if (right) right[k] = N[k].right;
+        }
+    });
+}
+
+int hmx_blocktree_traverse(const hmx_blocktree* b, int64_t* direct, int64_t* approx) {
+    return guarded([&] {
+        need(b && b->b, "hmx_blocktree_traverse: NULL block tree");
+        std::vector<std::pair<int64_t, int64_t>> d, a;
+        hmx::traverse(*b->b, &d, &a);
+        for (size_t k = 0; k < d.size() && direct; k++) {
+            direct[2 * k] = d[k].first;
+            direct[2 * k + 1] = d[k].second;
+        }
+        for (size_t k = 0; k < a.size() && approx; k++) {
+            approx[2 * k] = a[k].first;
+            approx[2 * k + 1] = a[k].second;
+        }
+    });
+}
+
+int hmx_blocktree_destroy(hmx_blocktree* b) {
+    delete b;
+    return HMX_OK;
+}
+
+// ---- low rank ----------------------------------------------------------------------
+int hmx_aca_plus(int64_t n_rows, int64_t n_cols, hmx_vec_fn calc_rows, hmx_vec_fn calc_cols,
+                 void* ctx, double eps, int64_t max_iter, hmx_lowrank** out) {
+    return guarded([&] {
+        need(out && calc_rows && calc_cols && n_rows > 0 && n_cols > 0, "hmx_aca_plus: bad argument");
+        *out = nullptr;
+        auto row = [&](int64_t I, double* o) {
+            int rc = calc_rows(ctx, I + 1, o);
+            if (rc) throw hmx::Error(HMX_ERR_CALLBACK, "calc_rows callback failed");
+        };
+        auto col = [&](int64_t J, double* o) {
+            int rc = calc_cols(ctx, J + 1, o);
+            if (rc) throw hmx::Error(HMX_ERR_CALLBACK, "calc_cols callback failed");
+        };
+        hmx::LowRank lr = hmx::aca_plus(n_rows, n_cols, row, col, eps, max_iter);
+        *out = new hmx_lowrank{std::move(lr)};
+    });
+}
+
+int hmx_lowrank_create(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
+                       hmx_lowrank** out) {
+    return guarded([&] {
+        need(out && m > 0 && n > 0 && k >= 0 && (k == 0 || (U && V)), "hmx_lowrank_create: bad argument");
+        *out = nullptr;
+        hmx::LowRank lr;
+        lr.m = m;
+        lr.n = n;
+        lr.k = k;
+        lr.U.assign(U, U + m * k);
+        lr.V.assign(V, V + k * n);
+        *out = new hmx_lowrank{std::move(lr)};
+    });
+}
+
+int hmx_lowrank_sizes(const hmx_lowrank* lr, int64_t* m, int64_t* n, int64_t* rank) {
+    return guarded([&] {
+        need(lr != nullptr, "hmx_lowrank_sizes: NULL");
+        if (m) *m = lr->lr.m;
+        if (n) *n = lr->lr.n;
+        if (rank) *rank = lr->lr.k;
+    });
+}
+
+int hmx_lowrank_export(const hmx_lowrank* lr, double* U, double* V) {
+    return guarded([&] {
+        need(lr != nullptr, "hmx_lowrank_export: NULL");
+        if (U) std::copy(lr->lr.U.begin(), lr->lr.U.end(), U);
+        if (V) std::copy(lr->lr.V.begin(), lr->lr.V.end(), V);
+    });
+}
+
+int hmx_svd_recompress(const hmx_lowrank* in, double eps, hmx_lowrank** out) {
+    return guarded([&] {
+        need(in && out, "hmx_svd_recompress: NULL");
+        *out = nullptr;
+        *out = new hmx_lowrank{hmx::svd_recompress(in->lr, eps)};
+    });
+}
+
+int hmx_lowrank_destroy(hmx_lowrank* lr) {
+    delete lr;
+    return HMX_OK;
+}
+
+// ---- H-matrix ------------------------------------------------------------------------
+int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                      const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
+                      void* ctx, double eta, double eps, int32_t svd_recompress,
+                      int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out) {
+    return guarded([&] {
+        need(target && source && target->t && source->t && out, "hmx_hmatrix_build: NULL argument");
+        *out = nullptr;
+        need(storage_mask >= 0 && storage_mask < 8, "hmx_hmatrix_build: bad storage_mask");
+        const hmx::ClusterTree& T = *target->t;
+        const hmx::ClusterTree& S = *source->t;
+        hmx::EntrySource src;
+        src.kernel_id = kernel_id;
+        std::vector<double> xt, xs;
+        if (kernel_id == HMX_KERNEL_CALLBACK) {
+            need(fill != nullptr, "hmx_hmatrix_build: callback kernel without fill function");
+            src.fill = fill;
+            src.ctx = ctx;
+        } else {
+            need(kernel_id == HMX_KERNEL_LAPLACE || kernel_id == HMX_KERNEL_LOG2D,
+                 "hmx_hmatrix_build: unknown kernel id");
+            need(target_xyz && source_xyz, "hmx_hmatrix_build: built-in kernels need coordinates");
+            if (T.dims != 1 || S.dims != 1)
+                throw hmx::Error(HMX_ERR_UNSUPPORTED, "built-in kernels are scalar (dims == 1)");
+            xt.assign(target_xyz, target_xyz + 3 * T.npts);
+            xs.assign(source_xyz, source_xyz + 3 * S.npts);
+            src.xt = xt.data();
+            src.xs = xs.data();
+        }
+        auto h = hmx::build_hmatrix(T, S, src, eta, eps, svd_recompress != 0, nthreads);
+        h->xt_copy = std::move(xt);
+        h->xs_copy = std::move(xs);
+        h->src.xt = h->xt_copy.empty() ? nullptr : h->xt_copy.data();
+        h->src.xs = h->xs_copy.empty() ? nullptr : h->xs_copy.data();
+        h->device = device;
+        for (int s = 0; s < 3; s++)
+            if (storage_mask & (1 << s)) hmx::device_build(*h, s, T);
+        *out = new hmx_hmatrix{std::move(h)};
+    });
+}
+
+int hmx_hmatrix_sizes(const hmx_hmatrix* hm, int64_t* s) {
+    return guarded([&] {
+        need(hm && hm->h && s, "hmx_hmatrix_sizes: NULL");
+        const hmx::HMatrix& h = *hm->h;
+        s[0] = h.m;
+        s[1] = h.n;
+        s[2] = static_cast<int64_t>(h.dense.size());
+        s[3] = static_cast<int64_t>(h.lowrank.size());
+        s[4] = h.R;
+        s[5] = h.dlen;
+        s[6] = static_cast<int64_t>(h.U.size());
+        s[7] = static_cast<int64_t>(h.V.size());
+    });
+}
+
+int hmx_hmatrix_export_reference_layout(const hmx_hmatrix* hm, int64_t* tmap, int64_t* smap,
+                                        int64_t* dense_idx, int64_t* U_idx, int64_t* V_idx,
+                                        double* D, double* U, double* V) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_hmatrix_export_reference_layout: NULL");
+        hmx::HMatrix& h = *hm->h;
+        if (tmap) std::copy(h.tmap.begin(), h.tmap.end(), tmap);
+        if (smap) std::copy(h.smap.begin(), h.smap.end(), smap);
+        if (dense_idx)
+            for (size_t b = 0; b < h.dense.size(); b++) {
+                const auto& d = h.dense[b];
+                int64_t* e = dense_idx + 5 * b;
+                e[0] = d.r0;
+                e[1] = d.r1;
+                e[2] = d.c0;
+                e[3] = d.c1;
+                e[4] = d.off;
+            }
+        if (U_idx)
+            for (size_t b = 0; b < h.lowrank.size(); b++) {
+                const auto& L = h.lowrank[b];
+                int64_t* e = U_idx + 5 * b;
+                e[0] = L.r0;
+                e[1] = L.r1;
+                e[2] = L.vx0 + 1;
+                e[3] = L.vx0 + L.k;
+                e[4] = L.uoff;
+            }
+        if (V_idx) {
+            int64_t c = 0;
+            for (const auto& L : h.lowrank) {
+                const int64_t nn = L.c1 - L.c0 + 1;
+                for (int64_t q = 0; q < L.k; q++, c++) {
+                    V_idx[3 * c] = L.c0;
+                    V_idx[3 * c + 1] = L.c1;
+                    V_idx[3 * c + 2] = L.voff + q * nn;
+                }
+            }
+        }
+        if (D) {
+            const_cast<hmx::HMatrix&>(h).ensure_dense_payload();
+            std::copy(h.D.begin(), h.D.end(), D);
+        }
+        if (U) std::copy(h.U.begin(), h.U.end(), U);
+        if (V) std::copy(h.V.begin(), h.V.end(), V);
+    });
+}
+
+int hmx_hmatrix_export_block_ranges(const hmx_hmatrix* hm, int64_t* dense_ranges,
+                                    int64_t* lowrank_ranges) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_hmatrix_export_block_ranges: NULL");
+        const hmx::HMatrix& h = *hm->h;
+        if (dense_ranges)
+            for (size_t b = 0; b < h.dense.size(); b++) {
+                const auto& d = h.dense[b];
+                int64_t* e = dense_ranges + 4 * b;
+                e[0] = d.r0;
+                e[1] = d.r1;
+                e[2] = d.c0;
+                e[3] = d.c1;
+            }
+        if (lowrank_ranges)
+            for (size_t b = 0; b < h.lowrank.size(); b++) {
+                const auto& L = h.lowrank[b];
+                int64_t* e = lowrank_ranges + 4 * b;
+                e[0] = L.r0;
+                e[1] = L.r1;
+                e[2] = L.c0;
+                e[3] = L.c1;
+            }
+    });
+}
+
+int hmx_hmatrix_info(const hmx_hmatrix* hm, double* out) {
+    return guarded([&] {
+        need(hm && hm->h && out, "hmx_hmatrix_info: NULL");
+        const hmx::HMatrix& h = *hm->h;
+        const double nd = static_cast<double>(h.dense.size());
+        const double na = static_cast<double>(h.lowrank.size());
+        int64_t kmin = 0, kmax = 0, dmin = 0, dmax = 0;
+        for (size_t b = 0; b < h.lowrank.size(); b++) {
+            int64_t k = h.lowrank[b].k;
+            kmin = b == 0 ? k : std::min(kmin, k);
+            kmax = b == 0 ? k : std::max(kmax, k);
+        }
+        for (size_t b = 0; b < h.dense.size(); b++) {
+            const auto& d = h.dense[b];
+            int64_t s = (d.r1 - d.r0 + 1) * (d.c1 - d.c0 + 1);
+            dmin = b == 0 ? s : std::min(dmin, s);
+            dmax = b == 0 ? s : std::max(dmax, s);
+        }
+        const double nnzD = static_cast<double>(h.dlen), nnzU = static_cast<double>(h.U.size()),
+                     nnzV = static_cast<double>(h.V.size());
+        out[0] = nd + na;
+        out[1] = na;
+        out[2] = nd;
+        out[3] = static_cast<double>(kmin);
+        out[4] = static_cast<double>(kmax);
+        out[5] = static_cast<double>(dmin);
+        out[6] = static_cast<double>(dmax);
+        out[7] = static_cast<double>(h.m) * static_cast<double>(h.n) / (nnzD + nnzU + nnzV);
+        out[8] = nnzD;
+        out[9] = nnzU;
+        out[10] = nnzV;
+        int64_t idx = 0;
+        for (int s = 0; s < 3; s++) {
+            out[11 + s] = static_cast<double>(hmx::device_panel_bytes(h.dev[s].get()));
+            idx += hmx::device_index_bytes(h.dev[s].get());
+        }
+        out[14] = static_cast<double>(idx);
+        out[15] = h.build_seconds;
+    });
+}
+
+int hmx_hmatrix_destroy(hmx_hmatrix* h) {
+    return guarded([&] { delete h; });
+}
+
+int hmx_matvec(hmx_hmatrix* hm, int32_t storage, const void* X, int64_t ldx, void* Y, int64_t ldy,
+               int32_t nrhs, int32_t io_dtype, void* stream) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_matvec: NULL H-matrix");
+        hmx::device_matvec(*hm->h, storage, X, ldx, Y, ldy, nrhs, io_dtype, stream);
+    });
+}
+
+int hmx_profile_enable(hmx_hmatrix* hm, int32_t enable) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_profile_enable: NULL");
+        hmx::device_profile_enable(*hm->h, enable != 0);
+    });
+}
+
+int hmx_profile_read(hmx_hmatrix* hm, double* ms, int64_t* launches, int32_t reset) {
+    return guarded([&] {
+        need(hm && hm->h && ms && launches, "hmx_profile_read: NULL");
+        hmx::device_profile_read(*hm->h, ms, launches, reset != 0);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// hmx_host.hpp -- internal host-side structures of libhmx (not part of the ABI).
+//
+// Host code follows Julia 1.10 Float64 semantics wherever the reference's
+// results must be reproduced bit for bit (cluster tree, block tree, ACA+); the
+// whole host side is compiled with -ffp-contract=off.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hmx.h"
+
+namespace hmx {
+
+// Error carrying an hmx_status code; converted to a return code + thread-local
+// message at the C boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---- cluster tree (tree.jl:6-20) ------------------------------------------
+struct ClusterNode {
+    int64_t start;  // 1-based
+    int64_t end;    // 1-based, exclusive
+    double center[3];
+    double radius;
+    int64_t left, right;  // node ids, -1 for none
+    bool leaf;
+};
+
+struct ClusterTree {
+    int64_t npts = 0, dims = 1;
+    std::vector<int64_t> index_map;  // 1-based, length npts*dims
+    std::vector<ClusterNode> nodes;  // pre-order; node 0 is the root
+    std::vector<int64_t> leaves;     // leaf node ids in increasing start order
+};
+
+std::unique_ptr<ClusterTree> build_cluster_tree(const double* xyz, int64_t npts, int64_t leaf,
+                                                int64_t dims);
+void tree_info(const ClusterTree& t, int64_t* depth, int64_t* pmax, int64_t* pmin);
+
+// Julia LinearAlgebra.generic_norm2 for a 3-vector.
+double norm3(const double v[3]);
+
+// ---- block tree (block.jl:2-16) --------------------------------------------
+struct BlockNode {
+    int64_t t, s;        // cluster node ids
+    int64_t left, right; // block node ids, -1 for none
+    bool adm, dense;
+};
+
+struct BlockTree {
+    const ClusterTree* T = nullptr;
+    const ClusterTree* S = nullptr;
+    std::vector<BlockNode> nodes;  // node 0 is the root
+};
+
+std::unique_ptr<BlockTree> build_block_tree(const ClusterTree& T, const ClusterTree& S, double eta);
+void merge_dense(BlockTree& b);
+// DFS pre-order lists of (target node, source node)
+void traverse(const BlockTree& b, std::vector<std::pair<int64_t, int64_t>>* direct,
+              std::vector<std::pair<int64_t, int64_t>>* approx);
+
+// ---- entries -----------------------------------------------------------------
+// Evaluates K[rows, cols] (0-based original indices) into a column-major buffer.
+struct EntrySource {
+    int32_t kernel_id = 0;
+    const double* xt = nullptr;  // 3 x Nt
+    const double* xs = nullptr;  // 3 x Ns
+    hmx_block_fn fill = nullptr;
+    void* ctx = nullptr;
+    // scalar entry for built-in kernels
+    double entry(int64_t i, int64_t j) const;
+    // K[rows[0..nr), cols[0..nc)] -> out (column-major, nr x nc)
+    void block(const int64_t* rows, int64_t nr, const int64_t* cols, int64_t nc, double* out) const;
+    bool thread_safe() const { return kernel_id != HMX_KERNEL_CALLBACK; }
+};
+
+// ---- low rank (aca.jl) ----------------------------------------------------------
+struct LowRank {
+    int64_t m = 0, n = 0, k = 0;
+    std::vector<double> U;  // m x k column-major
+    std::vector<double> V;  // k x n column-major
+};
+
+// Row/column oracles for ACA+: fill residual-free matrix row I (n values) or
+// column J (m values); 0-based local indices.
+using RowFn = std::function<void(int64_t, double*)>;
+LowRank aca_plus(int64_t m, int64_t n, const RowFn& row, const RowFn& col, double eps,
+                 int64_t max_iter);
+LowRank svd_recompress(const LowRank& in, double eps);
+
+// ---- H-matrix -----------------------------------------------------------------
+struct DenseBlock {
+    int64_t r0, r1, c0, c1;  // 1-based inclusive (hmatrix.jl:185)
+    int64_t off;             // 0-based offset in the flattened dense payload
+};
+struct LowRankBlock {
+    int64_t r0, r1, c0, c1;  // 1-based inclusive
+    int64_t k;
+    int64_t uoff, voff;      // 0-based offsets in U_data / V_data
+    int64_t vx0;             // 0-based first row in Vx_buffer
+    int64_t snode;           // source cluster node id (groups the V*X work)
+};
+
+struct DeviceOp;  // device-resident operator for one storage precision
+void device_destroy(DeviceOp* op);
+struct DeviceOpDeleter {
+    void operator()(DeviceOp* op) const { device_destroy(op); }
+};
+
+struct HMatrix {
+    int64_t m = 0, n = 0;
+    std::vector<int64_t> tmap, smap;  // 1-based
+    std::vector<DenseBlock> dense;    // direct list, then storage fallbacks
+    std::vector<LowRankBlock> lowrank;
+    std::vector<double> U, V;         // flattened row-major payloads (FP64)
+    std::vector<double> D;            // flattened dense payload; filled only for
+                                      // callback kernels or on export
+    bool D_ready = false;
+    int64_t R = 0, dlen = 0;
+    EntrySource src;
+    std::vector<double> xt_copy, xs_copy;  // owned copies of the coordinates
+    std::vector<int64_t> target_leaf_start;  // 0-based start rows of target leaves (+ end)
+    double build_seconds = 0.0;
+    int32_t device = 0;
+    std::unique_ptr<DeviceOp, DeviceOpDeleter> dev[3];
+    void ensure_dense_payload();
+};
+
+std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& S,
+                                       const EntrySource& src, double eta, double eps,
+                                       bool svd, int nthreads);
+
+// Simple fixed-size worker pool: run f(i) for i in [0,n) on `threads` threads.
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f);
+int default_threads();
+
+// ---- device side (device.hip) -------------------------------------------------
+void device_build(HMatrix& h, int storage, const ClusterTree& T);
+void device_matvec(HMatrix& h, int storage, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                   int nrhs, int io_dtype, void* stream);
+int64_t device_panel_bytes(const DeviceOp* op);
+int64_t device_index_bytes(const DeviceOp* op);
+void device_profile_enable(HMatrix& h, bool on);
+void device_profile_read(HMatrix& h, double* ms, int64_t* launches, bool reset);
+
+}  // namespace hmx

@@ -1,0 +1,210 @@
+// host_hmatrix.cpp -- H-matrix construction on the host.
+//
+//   build_hmatrix    HMatrix(K, X, Y; eta, eps, svd_recompress), hmatrix.jl:72-148
+//                    with build_matrices, hmatrix.jl:170-217, and the flattened
+//                    reference layout of hmatrix.jl:95-147
+//   EntrySource      the user-matrix getindex contract (hmatrix.jl:183,195-196,209)
+//                    plus native built-in kernels
+//
+// Low-rank blocks are compressed in parallel (one block per task); each task is
+// a deterministic sequential FP64 computation, so the result does not depend on
+// the thread count.  Compiled with -ffp-contract=off.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+#include "hmx_host.hpp"
+
+namespace hmx {
+
+int default_threads() {
+    for (const char* var : {"HMX_NUM_THREADS", "OMP_NUM_THREADS"}) {
+        const char* s = std::getenv(var);
+        if (s && std::atoi(s) > 0) return std::atoi(s);
+    }
+    unsigned hc = std::thread::hardware_concurrency();
+    return hc ? static_cast<int>(hc) : 1;
+}
+
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f) {
+    if (threads <= 1 || n <= 1) {
+        for (int64_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&]() {
+        for (;;) {
+            int64_t i = next.fetch_add(1);
+            if (i >= n) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu);
+                if (!err) err = std::current_exception();
+                next.store(n);
+                return;
+            }
+        }
+    };
+    int t = static_cast<int>(std::min<int64_t>(threads, n));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < t; i++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+// ---- entries ---------------------------------------------------------------
+double EntrySource::entry(int64_t i, int64_t j) const {
+    const double* a = xt + 3 * i;
+    const double* b = xs + 3 * j;
+    if (kernel_id == HMX_KERNEL_LAPLACE) {
+        const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        double d2 = dx * dx + dy * dy;
+        d2 = d2 + dz * dz;
+        const double d = std::sqrt(d2);
+        return d == 0.0 ? 0.0 : 1.0 / d;
+    }
+    // HMX_KERNEL_LOG2D
+    double r[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    const double d = norm3(r);
+    return d < 1e-12 ? 1.0 : (-0.5 / 3.141592653589793) * std::log(d);
+}
+
+void EntrySource::block(const int64_t* rows, int64_t nr, const int64_t* cols, int64_t nc,
+                        double* out) const {
+    if (kernel_id != HMX_KERNEL_CALLBACK) {
+        for (int64_t j = 0; j < nc; j++)
+            for (int64_t i = 0; i < nr; i++) out[i + j * nr] = entry(rows[i], cols[j]);
+        return;
+    }
+    // The callback receives 1-based indices (Julia getindex).
+    std::vector<int64_t> r1(rows, rows + nr), c1(cols, cols + nc);
+    for (auto& v : r1) v += 1;
+    for (auto& v : c1) v += 1;
+    int rc = fill(ctx, r1.data(), nr, c1.data(), nc, out);
+    if (rc != 0) throw Error(HMX_ERR_CALLBACK, "entry callback returned " + std::to_string(rc));
+}
+
+// ---- H-matrix ----------------------------------------------------------------
+void HMatrix::ensure_dense_payload() {
+    if (D_ready) return;
+    D.assign(static_cast<size_t>(dlen), 0.0);
+    const int threads = src.thread_safe() ? default_threads() : 1;
+    parallel_for(static_cast<int64_t>(dense.size()), threads, [&](int64_t b) {
+        const DenseBlock& d = dense[b];
+        const int64_t mm = d.r1 - d.r0 + 1, nn = d.c1 - d.c0 + 1;
+        std::vector<int64_t> rows(mm), cols(nn);
+        for (int64_t i = 0; i < mm; i++) rows[i] = tmap[d.r0 - 1 + i] - 1;
+        for (int64_t j = 0; j < nn; j++) cols[j] = smap[d.c0 - 1 + j] - 1;
+        std::vector<double> cm(mm * nn);
+        src.block(rows.data(), mm, cols.data(), nn, cm.data());
+        double* out = D.data() + d.off;  // vec(D'): row-major
+        for (int64_t i = 0; i < mm; i++)
+            for (int64_t j = 0; j < nn; j++) out[i * nn + j] = cm[i + j * mm];
+    });
+    D_ready = true;
+}
+
+std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& S,
+                                       const EntrySource& src, double eta, double eps, bool svd,
+                                       int nthreads) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto h = std::make_unique<HMatrix>();
+    h->src = src;
+    h->m = static_cast<int64_t>(T.index_map.size());
+    h->n = static_cast<int64_t>(S.index_map.size());
+    h->tmap = T.index_map;
+    h->smap = S.index_map;
+
+    auto bt = build_block_tree(T, S, eta);  // hmatrix.jl:74
+    merge_dense(*bt);                       // :75
+    std::vector<std::pair<int64_t, int64_t>> direct, approx;
+    traverse(*bt, &direct, &approx);        // :78
+    bt.reset();
+
+    // Compress every admissible block (ACA+ at eps/10, then SVD at eps).
+    const int threads = src.thread_safe() ? (nthreads > 0 ? nthreads : default_threads()) : 1;
+    std::vector<LowRank> lr(approx.size());
+    parallel_for(static_cast<int64_t>(approx.size()), threads, [&](int64_t b) {
+        const ClusterNode& a = T.nodes[approx[b].first];
+        const ClusterNode& s = S.nodes[approx[b].second];
+        const int64_t mm = a.end - a.start, nn = s.end - s.start;
+        std::vector<int64_t> rows(mm), cols(nn);
+        for (int64_t i = 0; i < mm; i++) rows[i] = T.index_map[a.start - 1 + i] - 1;
+        for (int64_t j = 0; j < nn; j++) cols[j] = S.index_map[s.start - 1 + j] - 1;
+        RowFn rowf, colf;
+        if (src.kernel_id != HMX_KERNEL_CALLBACK) {
+            rowf = [&](int64_t I, double* out) {
+                for (int64_t j = 0; j < nn; j++) out[j] = src.entry(rows[I], cols[j]);
+            };
+            colf = [&](int64_t J, double* out) {
+                for (int64_t i = 0; i < mm; i++) out[i] = src.entry(rows[i], cols[J]);
+            };
+        } else {
+            rowf = [&](int64_t I, double* out) { src.block(&rows[I], 1, cols.data(), nn, out); };
+            colf = [&](int64_t J, double* out) { src.block(rows.data(), mm, &cols[J], 1, out); };
+        }
+        LowRank f = aca_plus(mm, nn, rowf, colf, eps / 10.0, 0);  // hmatrix.jl:194-196
+        if (svd) f = svd_recompress(f, eps);                       // :197-199
+        lr[b] = std::move(f);
+    });
+
+    // Storage test (m+n)k < mn, else the block is stored dense after all direct
+    // blocks, in approx order (hmatrix.jl:202-213).
+    int64_t off = 0;
+    auto add_dense = [&](const ClusterNode& a, const ClusterNode& s) {
+        DenseBlock d{a.start, a.end - 1, s.start, s.end - 1, off};
+        off += (a.end - a.start) * (s.end - s.start);
+        h->dense.push_back(d);
+    };
+    for (auto& [ti, si] : direct) add_dense(T.nodes[ti], S.nodes[si]);
+    int64_t ulen = 0, vlen = 0, vx = 0;
+    std::vector<int64_t> keep;
+    for (size_t b = 0; b < approx.size(); b++) {
+        const ClusterNode& a = T.nodes[approx[b].first];
+        const ClusterNode& s = S.nodes[approx[b].second];
+        const int64_t mm = a.end - a.start, nn = s.end - s.start, k = lr[b].k;
+        if (mm * k + k * nn < mm * nn) {
+            LowRankBlock L{a.start, a.end - 1, s.start, s.end - 1, k, ulen, vlen, vx,
+                           approx[b].second};
+            h->lowrank.push_back(L);
+            keep.push_back(static_cast<int64_t>(b));
+            ulen += mm * k;
+            vlen += nn * k;
+            vx += k;
+        } else {
+            add_dense(a, s);
+            lr[b] = LowRank();
+        }
+    }
+    h->dlen = off;
+    h->R = vx;
+    h->U.resize(static_cast<size_t>(ulen));
+    h->V.resize(static_cast<size_t>(vlen));
+    parallel_for(static_cast<int64_t>(keep.size()), threads, [&](int64_t q) {
+        LowRank& f = lr[keep[q]];
+        const LowRankBlock& L = h->lowrank[q];
+        double* U = h->U.data() + L.uoff;  // vec(U'): U row-major m x k
+        for (int64_t i = 0; i < f.m; i++)
+            for (int64_t c = 0; c < f.k; c++) U[i * f.k + c] = f.U[i + c * f.m];
+        double* V = h->V.data() + L.voff;  // vec(V'): V row-major k x n
+        for (int64_t c = 0; c < f.k; c++)
+            for (int64_t j = 0; j < f.n; j++) V[c * f.n + j] = f.V[c + j * f.k];
+        f = LowRank();  // release the column-major factors as soon as they are copied
+    });
+    for (int64_t l : T.leaves) h->target_leaf_start.push_back(T.nodes[l].start - 1);
+    h->target_leaf_start.push_back(h->m);
+    if (src.kernel_id == HMX_KERNEL_CALLBACK) h->ensure_dense_payload();
+    h->build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return h;
+}
+
+}  // namespace hmx

@@ -1,0 +1,253 @@
+// host_trees.cpp -- cluster tree and block tree (host, FP64, bit-exact with the
+// reference's Julia 1.10 evaluation; compiled with -ffp-contract=off).
+//
+//   build_cluster_tree   ClusterTree(...)            tree.jl:45-67
+//                        build_tree_node             tree.jl:70-116
+//                        update_tree_indices!        tree.jl:123-137
+//   tree_info            info(::ClusterTree)         tree.jl:153-187
+//   build_block_tree     BlockTree / build_block_tree!  block.jl:31-90
+//   merge_dense          merge_dense_matrices!       block.jl:98-112
+//   traverse             traverse                    block.jl:127-155
+#include <algorithm>
+#include <cmath>
+
+#include "hmx_host.hpp"
+
+namespace hmx {
+
+namespace {
+
+// Base.min / Base.max for Float64 in Julia 1.10 (signed zeros, NaN propagate).
+inline double jmin(double x, double y) {
+    double d = x - y;
+    double r = std::signbit(d) ? x : y;
+    return (std::isnan(x) || std::isnan(y)) ? d : r;
+}
+inline double jmax(double x, double y) {
+    double d = x - y;
+    double r = std::signbit(d) ? y : x;
+    return (std::isnan(x) || std::isnan(y)) ? d : r;
+}
+// Base.isless for Float64 (NaN is the largest value).
+inline bool jisless(double a, double b) {
+    if (std::isnan(a)) return false;
+    if (std::isnan(b)) return true;
+    if (a < b) return true;
+    return a == b && std::signbit(a) && !std::signbit(b);
+}
+
+}  // namespace
+
+double norm3(const double v[3]) {
+    // LinearAlgebra.generic_norm2: 3-element arrays stay below NRM2_CUTOFF.
+    double amax = jmax(jmax(std::fabs(v[0]), std::fabs(v[1])), std::fabs(v[2]));
+    if (amax == 0.0 || std::isinf(amax)) return amax;
+    if (std::isfinite(3.0 * amax * amax) && amax * amax != 0.0) {
+        double s = v[0] * v[0];
+        s += v[1] * v[1];
+        s += v[2] * v[2];
+        return std::sqrt(s);
+    }
+    double s = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double q = std::fabs(v[d]) / amax;
+        s = (d == 0) ? q * q : s + q * q;
+    }
+    return amax * std::sqrt(s);
+}
+
+std::unique_ptr<ClusterTree> build_cluster_tree(const double* xyz, int64_t npts, int64_t leaf,
+                                                int64_t dims) {
+    if (dims <= 0) throw Error(HMX_ERR_DIMS, "Parameter `dims` must be a positive integer.");
+    if (!xyz || npts <= 0) throw Error(HMX_ERR_ARG, "hmx_tree_build: empty point set");
+    auto t = std::make_unique<ClusterTree>();
+    t->npts = npts;
+    t->dims = dims;
+    std::vector<int64_t>& imap = t->index_map;
+    imap.resize(npts);
+    for (int64_t i = 0; i < npts; i++) imap[i] = i + 1;
+
+    // Explicit stack; children are pushed right-then-left so nodes are numbered
+    // in the same pre-order the recursive reference visits them.
+    struct Task {
+        int64_t start, end, parent;
+        int side;  // 0 = left child of parent, 1 = right child
+    };
+    std::vector<Task> stack{{1, npts + 1, -1, 0}};
+    t->nodes.reserve(2 * (npts / std::max<int64_t>(leaf, 1)) + 8);
+    while (!stack.empty()) {
+        Task tk = stack.back();
+        stack.pop_back();
+        int64_t cnt = tk.end - tk.start;
+        if (cnt <= 0)
+            throw Error(HMX_ERR_EMPTY_CLUSTER,
+                        "cluster split produced an empty cluster (coincident points exceed the "
+                        "leaf size)");
+        int64_t* v = imap.data() + (tk.start - 1);
+        const double* p0 = xyz + 3 * (v[0] - 1);
+        double lo[3] = {p0[0], p0[1], p0[2]}, hi[3] = {p0[0], p0[1], p0[2]};
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (int64_t p = 0; p < cnt; p++) {
+            const double* q = xyz + 3 * (v[p] - 1);
+            for (int d = 0; d < 3; d++) {
+                lo[d] = jmin(lo[d], q[d]);
+                hi[d] = jmax(hi[d], q[d]);
+                sum[d] = sum[d] + q[d];  // sequential column sum, tree.jl:85
+            }
+        }
+        int axis = 0;  // argmax of the box lengths: first maximum, tree.jl:81
+        double len0 = hi[0] - lo[0];
+        for (int d = 1; d < 3; d++) {
+            double ld = hi[d] - lo[d];
+            if (jisless(len0, ld)) {
+                axis = d;
+                len0 = ld;
+            }
+        }
+        ClusterNode nd;
+        nd.start = tk.start;
+        nd.end = tk.end;
+        for (int d = 0; d < 3; d++) nd.center[d] = sum[d] / static_cast<double>(cnt);
+        double a[3], b[3];
+        for (int d = 0; d < 3; d++) {
+            a[d] = nd.center[d] - hi[d];
+            b[d] = nd.center[d] - lo[d];
+        }
+        nd.radius = jmax(norm3(a), norm3(b));  // tree.jl:86-87
+        nd.left = nd.right = -1;
+        nd.leaf = cnt <= leaf;                 // tree.jl:90
+        int64_t id = static_cast<int64_t>(t->nodes.size());
+        t->nodes.push_back(nd);
+        if (tk.parent >= 0) {
+            if (tk.side == 0)
+                t->nodes[tk.parent].left = id;
+            else
+                t->nodes[tk.parent].right = id;
+        }
+        if (nd.leaf) continue;
+        // Stable partition on coordinate < centroid (strict), tree.jl:96-105.
+        const double cut = nd.center[axis];
+        int64_t* mid = std::stable_partition(
+            v, v + cnt, [&](int64_t p) { return xyz[3 * (p - 1) + axis] < cut; });
+        int64_t split = tk.start + (mid - v);
+        stack.push_back({split, tk.end, id, 1});
+        stack.push_back({tk.start, split, id, 0});
+    }
+    if (dims > 1) {
+        std::vector<int64_t> e(static_cast<size_t>(npts * dims));
+        for (int64_t p = 0; p < npts; p++)
+            for (int64_t i = 1; i <= dims; i++) e[p * dims + (i - 1)] = dims * imap[p] - (dims - i);
+        imap.swap(e);
+        for (auto& nd : t->nodes) {
+            int64_t s = nd.start;
+            nd.start = dims * (s - 1) + 1;
+            nd.end = dims * (nd.end - s) + nd.start;
+        }
+    }
+    for (int64_t k = 0; k < static_cast<int64_t>(t->nodes.size()); k++)
+        if (t->nodes[k].leaf) t->leaves.push_back(k);  // pre-order == increasing start
+    return t;
+}
+
+void tree_info(const ClusterTree& t, int64_t* depth, int64_t* pmax, int64_t* pmin) {
+    int64_t dmax = 0, mx = 0, mn = -1;
+    std::vector<std::pair<int64_t, int64_t>> st{{0, 1}};
+    while (!st.empty()) {
+        auto [id, d] = st.back();
+        st.pop_back();
+        const ClusterNode& nd = t.nodes[id];
+        dmax = std::max(dmax, d);
+        if (nd.leaf) {
+            int64_t c = nd.end - nd.start;
+            mx = std::max(mx, c);
+            mn = (mn < 0) ? c : std::min(mn, c);
+        }
+        if (nd.right >= 0) st.push_back({nd.right, d + 1});
+        if (nd.left >= 0) st.push_back({nd.left, d + 1});
+    }
+    *depth = dmax;
+    *pmax = mx;
+    *pmin = mn < 0 ? 0 : mn;
+}
+
+std::unique_ptr<BlockTree> build_block_tree(const ClusterTree& T, const ClusterTree& S,
+                                            double eta) {
+    auto b = std::make_unique<BlockTree>();
+    b->T = &T;
+    b->S = &S;
+    b->nodes.push_back({0, 0, -1, -1, false, false});
+    // Node ids follow the reference's allocation order: both children of a
+    // node are created together, then the left subtree is built before the
+    // right one (build_block_tree! recursion, block.jl:85-88).
+    std::vector<int64_t> stack{0};
+    while (!stack.empty()) {
+        int64_t id = stack.back();
+        stack.pop_back();
+        const ClusterNode& X = T.nodes[b->nodes[id].t];
+        const ClusterNode& Y = S.nodes[b->nodes[id].s];
+        double d[3] = {X.center[0] - Y.center[0], X.center[1] - Y.center[1],
+                       X.center[2] - Y.center[2]};
+        double dist = norm3(d);
+        if (dist > eta * (X.radius + Y.radius)) {  // block.jl:63-66
+            b->nodes[id].adm = true;
+            continue;
+        }
+        if (X.leaf && Y.leaf) {  // block.jl:70-72
+            b->nodes[id].dense = true;
+            continue;
+        }
+        bool split_Y = (!Y.leaf && Y.radius > X.radius) || X.leaf;  // block.jl:75
+        int64_t tid = b->nodes[id].t, sid = b->nodes[id].s;
+        int64_t l = static_cast<int64_t>(b->nodes.size());
+        if (split_Y) {
+            b->nodes.push_back({tid, Y.left, -1, -1, false, false});
+            b->nodes.push_back({tid, Y.right, -1, -1, false, false});
+        } else {
+            b->nodes.push_back({X.left, sid, -1, -1, false, false});
+            b->nodes.push_back({X.right, sid, -1, -1, false, false});
+        }
+        b->nodes[id].left = l;
+        b->nodes[id].right = l + 1;
+        stack.push_back(l + 1);
+        stack.push_back(l);
+    }
+    return b;
+}
+
+void merge_dense(BlockTree& b) {
+    // Post-order: a node with two children is dense iff both children are.
+    std::vector<std::pair<int64_t, bool>> st{{0, false}};
+    while (!st.empty()) {
+        auto [id, visited] = st.back();
+        st.pop_back();
+        BlockNode& nd = b.nodes[id];
+        if (nd.left < 0 || nd.right < 0) continue;
+        if (!visited) {
+            st.push_back({id, true});
+            st.push_back({nd.right, false});
+            st.push_back({nd.left, false});
+        } else {
+            nd.dense = b.nodes[nd.left].dense && b.nodes[nd.right].dense;
+        }
+    }
+}
+
+void traverse(const BlockTree& b, std::vector<std::pair<int64_t, int64_t>>* direct,
+              std::vector<std::pair<int64_t, int64_t>>* approx) {
+    std::vector<int64_t> st{0};
+    while (!st.empty()) {
+        int64_t id = st.back();
+        st.pop_back();
+        const BlockNode& nd = b.nodes[id];
+        if (nd.adm)
+            approx->push_back({nd.t, nd.s});
+        else if (nd.dense)
+            direct->push_back({nd.t, nd.s});
+        if (!nd.dense && !nd.adm) {
+            if (nd.right >= 0) st.push_back(nd.right);
+            if (nd.left >= 0) st.push_back(nd.left);
+        }
+    }
+}
+
+}  // namespace hmx

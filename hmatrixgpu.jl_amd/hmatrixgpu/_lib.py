@@ -1,0 +1,101 @@
+"""ctypes binding of libhmx.so (the C ABI declared in include/hmx.h).
+
+The library is loaded from the package tree (``hmatrixgpu.jl_amd/lib``); there
+is no fallback implementation: if the shared library is missing the import
+fails loudly and tells the caller how to build it.
+"""
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("HMX_LIBRARY", os.path.join(PKG_ROOT, "lib", "libhmx.so"))
+
+HMX_OK = 0
+HMX_F64, HMX_F32, HMX_BF16 = 0, 1, 2
+HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D = 0, 1, 2
+
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+i8p = C.POINTER(C.c_int8)
+vp = C.c_void_p
+
+BLOCK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, i64p, C.c_int64, i64p, C.c_int64, f64p)
+VEC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, f64p)
+
+# name -> (restype, argtypes); every function returns an int status
+SIGNATURES = {
+    "hmx_last_error": (C.c_char_p, []),
+    "hmx_version": (C.c_int, [C.POINTER(C.c_int32)] * 3),
+    "hmx_tree_build": (C.c_int, [f64p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(vp)]),
+    "hmx_tree_sizes": (C.c_int, [vp, i64p, i64p]),
+    "hmx_tree_export": (C.c_int, [vp, i64p, i64p, i64p, f64p, f64p, i64p, i64p, i8p]),
+    "hmx_tree_info": (C.c_int, [vp, i64p, i64p, i64p]),
+    "hmx_tree_destroy": (C.c_int, [vp]),
+    "hmx_blocktree_build": (C.c_int, [vp, vp, C.c_double, C.POINTER(vp)]),
+    "hmx_blocktree_merge_dense": (C.c_int, [vp]),
+    "hmx_blocktree_sizes": (C.c_int, [vp, i64p, i64p, i64p]),
+    "hmx_blocktree_export": (C.c_int, [vp, i64p, i64p, i8p, i8p, i64p, i64p]),
+    "hmx_blocktree_traverse": (C.c_int, [vp, i64p, i64p]),
+    "hmx_blocktree_destroy": (C.c_int, [vp]),
+    "hmx_aca_plus": (C.c_int, [C.c_int64, C.c_int64, VEC_FN, VEC_FN, vp, C.c_double, C.c_int64,
+                               C.POINTER(vp)]),
+    "hmx_lowrank_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, f64p, f64p, C.POINTER(vp)]),
+    "hmx_lowrank_sizes": (C.c_int, [vp, i64p, i64p, i64p]),
+    "hmx_lowrank_export": (C.c_int, [vp, f64p, f64p]),
+    "hmx_svd_recompress": (C.c_int, [vp, C.c_double, C.POINTER(vp)]),
+    "hmx_lowrank_destroy": (C.c_int, [vp]),
+    "hmx_hmatrix_build": (C.c_int, [vp, vp, C.c_int32, f64p, f64p, BLOCK_FN, vp, C.c_double,
+                                    C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(vp)]),
+    "hmx_hmatrix_sizes": (C.c_int, [vp, i64p]),
+    "hmx_hmatrix_export_reference_layout": (C.c_int, [vp, i64p, i64p, i64p, i64p, i64p, f64p,
+                                                      f64p, f64p]),
+    "hmx_hmatrix_export_block_ranges": (C.c_int, [vp, i64p, i64p]),
+    "hmx_hmatrix_info": (C.c_int, [vp, f64p]),
+    "hmx_hmatrix_destroy": (C.c_int, [vp]),
+    "hmx_matvec": (C.c_int, [vp, C.c_int32, vp, C.c_int64, vp, C.c_int64, C.c_int32, C.c_int32,
+                             vp]),
+    "hmx_profile_enable": (C.c_int, [vp, C.c_int32]),
+    "hmx_profile_read": (C.c_int, [vp, f64p, i64p, C.c_int32]),
+}
+
+
+class HMXError(RuntimeError):
+    """Raised when a libhmx call returns a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__("libhmx error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libhmx.so not found at %s; build it with `make -C hmatrixgpu.jl_amd` "
+            "(or __graft_entry__.build())" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status):
+    if status != HMX_OK:
+        raise HMXError(status, lib.hmx_last_error().decode())
+
+
+def ptr_f64(a):
+    return a.ctypes.data_as(f64p)
+
+
+def ptr_i64(a):
+    return a.ctypes.data_as(i64p)
+
+
+def ptr_i8(a):
+    return a.ctypes.data_as(i8p)

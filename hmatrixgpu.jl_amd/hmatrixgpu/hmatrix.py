@@ -1,0 +1,365 @@
+"""H-matrix construction and product -- the Python mirror of hmatrix.jl and
+mult.jl on top of libhmx.
+
+    HMatrix(K, X, Y; eta=1.5, eps=1e-5, flatten=true, index_map_using_cpu=true,
+            svd_recompress=true)                          hmatrix.jl:72-148
+    mul_(result, H, x)   (Julia: mul!)                    mult.jl:159-189
+    H * x, H @ x                                          mult.jl:203-209
+    info(H)                                               hmatrix.jl:289-330
+
+K is either a built-in kernel (LaplaceKernel, Log2DKernel: evaluated natively
+on host and GPU) or any matrix-like object with ``shape`` and numpy-style
+``K[rows, cols]`` indexing on 0-based integer arrays (entries are fetched
+through a host callback during the build).
+
+The product always runs on the GPU through the HIP kernels of libhmx; there is
+no CPU product.  ``flatten=False`` returns an HMatrixCPU whose fields mirror
+the reference's HMatrixCPU (per-block matrices); its product is the same GPU
+product.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+STORAGE = {"f64": L.HMX_F64, "fp64": L.HMX_F64, "float64": L.HMX_F64,
+           "f32": L.HMX_F32, "fp32": L.HMX_F32, "float32": L.HMX_F32,
+           "bf16": L.HMX_BF16, "bfloat16": L.HMX_BF16}
+
+
+class _PointKernel:
+    kernel_id = None
+
+    def __init__(self, target_points, source_points=None):
+        self.X = np.asarray(target_points, dtype=np.float64)
+        self.Y = self.X if source_points is None else np.asarray(source_points, dtype=np.float64)
+        if self.X.shape[0] != 3 or self.Y.shape[0] != 3:
+            raise ValueError("points must have shape (3, N)")
+        self.shape = (self.X.shape[1], self.Y.shape[1])
+        self.dtype = np.dtype(np.float64)
+
+    def __getitem__(self, idx):
+        rows, cols = idx
+        rows = np.atleast_1d(np.asarray(rows))
+        cols = np.atleast_1d(np.asarray(cols))
+        a = self.X[:, rows].T[:, None, :]
+        b = self.Y[:, cols].T[None, :, :]
+        return self._eval(a - b)
+
+
+class LaplaceKernel(_PointKernel):
+    """K[i, j] = 1/|x_i - y_j|, 0 where the points coincide (the diagonal)."""
+
+    kernel_id = L.HMX_KERNEL_LAPLACE
+
+    @staticmethod
+    def _eval(r):
+        d = np.sqrt((r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1]) + r[..., 2] * r[..., 2])
+        with np.errstate(divide="ignore"):
+            return np.where(d == 0.0, 0.0, 1.0 / d)
+
+
+class Log2DKernel(_PointKernel):
+    """K[i, j] = -log(d)/(2 pi), 1.0 where d < 1e-12 (test_hmatrix.jl:17-20)."""
+
+    kernel_id = L.HMX_KERNEL_LOG2D
+
+    @staticmethod
+    def _eval(r):
+        d = np.sqrt((r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1]) + r[..., 2] * r[..., 2])
+        with np.errstate(divide="ignore"):
+            return np.where(d < 1e-12, 1.0, (-0.5 / math.pi) * np.log(np.maximum(d, 1e-300)))
+
+
+def _block_callback(K, errors):
+    def cb(ctx, rows, nr, cols, nc, out):
+        try:
+            r = np.ctypeslib.as_array(rows, shape=(nr,)) - 1
+            c = np.ctypeslib.as_array(cols, shape=(nc,)) - 1
+            blk = np.asarray(K[r[:, None], c[None, :]], dtype=np.float64).reshape(nr, nc)
+            o = np.ctypeslib.as_array(out, shape=(nc, nr))  # column-major nr x nc
+            o[:, :] = blk.T
+            return 0
+        except Exception as e:  # noqa: BLE001 - re-raised after the library call
+            errors.append(e)
+            return 1
+    return L.BLOCK_FN(cb)
+
+
+class HMatrix:
+    """Flattened H-matrix (hmatrix.jl:46-57) resident on the GPU.  Like the
+    reference constructor, ``HMatrix(..., flatten=False)`` returns an
+    HMatrixCPU."""
+
+    def __new__(cls, K=None, X=None, Y=None, eta=1.5, eps=1e-5, flatten=True, **kw):
+        if cls is HMatrix and not flatten:
+            return super().__new__(HMatrixCPU)
+        return super().__new__(cls)
+
+    def __init__(self, K, X, Y, eta=1.5, eps=1e-5, flatten=True, index_map_using_cpu=True,
+                 svd_recompress=True, storage=("f64",), device=0, nthreads=0):
+        self.K = K
+        self.X, self.Y = X, Y
+        self.eta, self.eps = float(eta), float(eps)
+        self.flatten = bool(flatten)
+        mask = 0
+        for s in ([storage] if isinstance(storage, str) else storage):
+            mask |= 1 << STORAGE[s.lower()]
+        self.storage_mask = mask
+        self.device = int(device)
+        h = L.vp()
+        errors = []
+        kid = getattr(K, "kernel_id", None)
+        if kid is not None:
+            xt = np.ascontiguousarray(K.X.T)
+            ys = np.ascontiguousarray(K.Y.T)
+            cb = L.BLOCK_FN()
+            st = L.lib.hmx_hmatrix_build(X.handle, Y.handle, kid, L.ptr_f64(xt), L.ptr_f64(ys), cb,
+                                         None, self.eta, self.eps, int(bool(svd_recompress)), mask,
+                                         self.device, int(nthreads), C.byref(h))
+        else:
+            cb = _block_callback(K, errors)
+            st = L.lib.hmx_hmatrix_build(X.handle, Y.handle, L.HMX_KERNEL_CALLBACK, None, None, cb,
+                                         None, self.eta, self.eps, int(bool(svd_recompress)), mask,
+                                         self.device, int(nthreads), C.byref(h))
+        if errors:
+            raise errors[0]
+        L.check(st)
+        self._h = h
+        s = np.zeros(8, np.int64)
+        L.check(L.lib.hmx_hmatrix_sizes(h, L.ptr_i64(s)))
+        (self.m, self.n, self.n_dense, self.n_lowrank, self.R, self._dlen, self._ulen,
+         self._vlen) = [int(v) for v in s]
+        self._layout = None
+        self._scratch = {}
+
+    # ---- the reference's flattened fields (hmatrix.jl:46-57, 95-147) -------------
+    def layout(self, with_dense=True):
+        if self._layout is not None and (not with_dense or self._layout["dense_blocks"] is not None):
+            return self._layout
+        tm = np.zeros(self.m, np.int64)
+        sm = np.zeros(self.n, np.int64)
+        di = np.zeros((max(self.n_dense, 1), 5), np.int64)
+        ui = np.zeros((max(self.n_lowrank, 1), 5), np.int64)
+        vi = np.zeros((max(self.R, 1), 3), np.int64)
+        D = np.zeros(max(self._dlen, 1)) if with_dense else None
+        U = np.zeros(max(self._ulen, 1))
+        V = np.zeros(max(self._vlen, 1))
+        L.check(L.lib.hmx_hmatrix_export_reference_layout(
+            self._h, L.ptr_i64(tm), L.ptr_i64(sm), L.ptr_i64(di), L.ptr_i64(ui), L.ptr_i64(vi),
+            L.ptr_f64(D) if with_dense else None, L.ptr_f64(U), L.ptr_f64(V)))
+        self._layout = {
+            "target_index_map": tm, "source_index_map": sm,
+            # Julia stores these as 5 x n / 3 x R Int64 matrices (one column per block)
+            "dense_block_indices": di[:self.n_dense].T.copy(),
+            "U_block_indices": ui[:self.n_lowrank].T.copy(),
+            "V_block_indices": vi[:self.R].T.copy(),
+            "dense_blocks": D[:self._dlen] if with_dense else None,
+            "U_matrices": U[:self._ulen], "V_matrices": V[:self._vlen],
+        }
+        return self._layout
+
+    target_index_map = property(lambda self: self.layout(False)["target_index_map"])
+    source_index_map = property(lambda self: self.layout(False)["source_index_map"])
+    dense_block_indices = property(lambda self: self.layout(False)["dense_block_indices"])
+    U_block_indices = property(lambda self: self.layout(False)["U_block_indices"])
+    V_block_indices = property(lambda self: self.layout(False)["V_block_indices"])
+    dense_blocks = property(lambda self: self.layout(True)["dense_blocks"])
+    U_matrices = property(lambda self: self.layout(False)["U_matrices"])
+    V_matrices = property(lambda self: self.layout(False)["V_matrices"])
+
+    @property
+    def Vx_buffer(self):
+        """Length-R intermediate of the reference (hmatrix.jl:56); the device keeps
+        its own per-stream scratch, this host array only mirrors the size."""
+        return np.zeros(self.R)
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    def info(self):
+        o = np.zeros(16)
+        L.check(L.lib.hmx_hmatrix_info(self._h, L.ptr_f64(o)))
+        return {
+            "data_type": "Float64", "size": (self.m, self.n),
+            "leaves": int(o[0]), "admissible_leaves": int(o[1]), "full_leaves": int(o[2]),
+            "min_rank": int(o[3]), "max_rank": int(o[4]), "min_dense_size": int(o[5]),
+            "max_dense_size": int(o[6]), "compression_ratio": float(o[7]),
+            "nnz_D": int(o[8]), "nnz_U": int(o[9]), "nnz_V": int(o[10]),
+            "device_bytes": {"f64": int(o[11]), "f32": int(o[12]), "bf16": int(o[13])},
+            "device_index_bytes": int(o[14]), "build_seconds": float(o[15]),
+        }
+
+    # ---- product ---------------------------------------------------------------
+    def default_storage(self):
+        for s in (L.HMX_F64, L.HMX_F32, L.HMX_BF16):
+            if self.storage_mask & (1 << s):
+                return s
+        raise RuntimeError("this H-matrix has no device storage (storage=())")
+
+    def matvec_ptr(self, X_ptr, ldx, Y_ptr, ldy, nrhs, io_dtype, stream=0, storage=None):
+        """Raw device-pointer product (hmx_matvec)."""
+        s = self.default_storage() if storage is None else STORAGE.get(storage, storage)
+        L.check(L.lib.hmx_matvec(self._h, int(s), C.c_void_p(X_ptr), int(ldx), C.c_void_p(Y_ptr),
+                                 int(ldy), int(nrhs), int(io_dtype), C.c_void_p(stream)))
+
+    def __mul__(self, x):
+        return product(self, x)
+
+    def __matmul__(self, x):
+        return product(self, x)
+
+    def profile(self, enable=True):
+        L.check(L.lib.hmx_profile_enable(self._h, int(bool(enable))))
+
+    def profile_read(self, reset=True):
+        ms = np.zeros(4)
+        n = np.zeros(4, np.int64)
+        L.check(L.lib.hmx_profile_read(self._h, L.ptr_f64(ms), L.ptr_i64(n), int(bool(reset))))
+        return ms, n
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib.hmx_hmatrix_destroy(h)
+            self._h = None
+
+
+class HMatrixCPU(HMatrix):
+    """flatten=False result (hmatrix.jl:18-27, 90-93): per-block matrices.  The
+    product is the same GPU product as HMatrix's."""
+
+    def _ranges(self):
+        if not hasattr(self, "_rg"):
+            d = np.zeros((max(self.n_dense, 1), 4), np.int64)
+            a = np.zeros((max(self.n_lowrank, 1), 4), np.int64)
+            L.check(L.lib.hmx_hmatrix_export_block_ranges(self._h, L.ptr_i64(d), L.ptr_i64(a)))
+            self._rg = ([tuple(int(v) for v in r) for r in d[:self.n_dense]],
+                        [tuple(int(v) for v in r) for r in a[:self.n_lowrank]])
+        return self._rg
+
+    @property
+    def dense_block_indices(self):
+        return self._ranges()[0]
+
+    @property
+    def approx_block_indices(self):
+        return self._ranges()[1]
+
+    @property
+    def dense_blocks(self):
+        lay = self.layout(True)
+        di = lay["dense_block_indices"]
+        out = []
+        for b in range(di.shape[1]):
+            r0, r1, c0, c1, off = [int(v) for v in di[:, b]]
+            mm, nn = r1 - r0 + 1, c1 - c0 + 1
+            out.append(lay["dense_blocks"][off:off + mm * nn].reshape(mm, nn))
+        return out
+
+    @property
+    def U_matrices(self):
+        lay = self.layout(False)
+        ui = lay["U_block_indices"]
+        out = []
+        for b in range(ui.shape[1]):
+            r0, r1, v0, v1, off = [int(v) for v in ui[:, b]]
+            mm, k = r1 - r0 + 1, v1 - v0 + 1
+            out.append(lay["U_matrices"][off:off + mm * k].reshape(mm, k))
+        return out
+
+    @property
+    def V_matrices(self):
+        lay = self.layout(False)
+        ui, vi = lay["U_block_indices"], lay["V_block_indices"]
+        out = []
+        for b, (_, _, c0, c1) in enumerate(self.approx_block_indices):
+            v0, v1 = int(ui[2, b]), int(ui[3, b])
+            k, nn = v1 - v0 + 1, c1 - c0 + 1
+            if k == 0:
+                out.append(np.zeros((0, nn)))
+                continue
+            off = int(vi[2, v0 - 1])
+            out.append(lay["V_matrices"][off:off + k * nn].reshape(k, nn))
+        return out
+
+
+def _torch():
+    import torch  # noqa: PLC0415 - torch is only needed for device memory
+    return torch
+
+
+def mul_(result, H, x, storage=None):
+    """mul!(result, H, x) (mult.jl:159-189): result[:, c] = H * x[:, c].
+
+    result / x: torch CUDA tensors (float64 or float32; vectors or column-major
+    n x nrhs matrices, i.e. transposed-contiguous 2-D tensors) -- the product is
+    asynchronous on the current stream.  numpy arrays are copied to the device
+    and back (synchronous)."""
+    torch = _torch()
+    if isinstance(x, np.ndarray) or isinstance(result, np.ndarray):
+        dev = torch.device("cuda", H.device)
+        xt = torch.as_tensor(np.asarray(x), device=dev)
+        yt = torch.empty(tuple(np.shape(result)), dtype=xt.dtype, device=dev)
+        if xt.dim() == 2:
+            xt = xt.t().contiguous().t()
+            yt = yt.t().contiguous().t()
+        mul_(yt, H, xt, storage=storage)
+        np.copyto(result, yt.cpu().numpy())
+        return result
+    if not (x.is_cuda and result.is_cuda):
+        raise ValueError("mul_: x and result must be device tensors or numpy arrays")
+    if x.dtype != result.dtype or x.dtype not in (torch.float64, torch.float32):
+        raise TypeError("mul_: x and result must share dtype float64 or float32")
+    io = L.HMX_F64 if x.dtype == torch.float64 else L.HMX_F32
+    if x.dim() == 1:
+        nrhs, ldx, ldy = 1, x.shape[0], result.shape[0]
+        if x.stride(0) != 1 or result.stride(0) != 1:
+            raise ValueError("mul_: vectors must be contiguous")
+    else:
+        nrhs = x.shape[1]
+        if x.stride(0) != 1 or result.stride(0) != 1:
+            raise ValueError("mul_: matrices must be column-major (stride(0) == 1)")
+        ldx, ldy = x.stride(1), result.stride(1)
+        if result.shape[1] != nrhs:
+            raise ValueError("mul_: column count mismatch")
+    if x.shape[0] != H.n or result.shape[0] != H.m:
+        raise ValueError("mul_: size mismatch (H is %d x %d)" % (H.m, H.n))
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    H.matvec_ptr(x.data_ptr(), max(ldx, H.n), result.data_ptr(), max(ldy, H.m), nrhs, io,
+                 stream, storage)
+    return result
+
+
+def product(H, x, storage=None):
+    """H * x (mult.jl:203-209): allocates the result and calls mul_."""
+    torch = _torch()
+    if isinstance(x, np.ndarray):
+        y = np.zeros((H.m,) + tuple(x.shape[1:]), dtype=x.dtype)
+        return mul_(y, H, x, storage=storage)
+    if x.dim() == 1:
+        y = torch.empty(H.m, dtype=x.dtype, device=x.device)
+    else:
+        y = torch.empty((x.shape[1], H.m), dtype=x.dtype, device=x.device).t()
+    return mul_(y, H, x, storage=storage)
+
+
+# ---- backend registry (HMatrixGPU.jl:14-109) ------------------------------------
+_groupsize = [512]
+
+
+def set_groupsize(x):
+    """set_groupsize (HMatrixGPU.jl:18-20).  Accepted for API compatibility: the
+    gfx950 kernels choose their own workgroup shape (4 waves of 64 lanes)."""
+    _groupsize[0] = int(x)
+    return _groupsize[0]
+
+
+def set_backend(backend="amd"):
+    """set_backend (HMatrixGPU.jl:55-87).  This engine has exactly one backend,
+    the MI355X HIP path; "amd"/"roc"/"amdgpu" select it, anything else is
+    refused (returns False)."""
+    return str(backend).lower() in ("amd", "roc", "amdgpu", "hip", "gfx950")

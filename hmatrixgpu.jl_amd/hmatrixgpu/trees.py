@@ -1,0 +1,266 @@
+"""Cluster tree, block tree and low-rank compression -- the Python mirror of
+HMatrixGPU.jl's tree.jl, block.jl and aca.jl on top of libhmx.
+
+Names, argument meaning and field names follow the reference so that code
+written against it reads the same:
+    ClusterTree(coordinates; max_points_per_leaf=32, dims=1)   tree.jl:45
+    info(tree)                                                 tree.jl:153
+    BlockTree(X, Y; eta=1.5, index_map_using_cpu=true)         block.jl:31
+    merge_dense_matrices_(node)                                block.jl:98
+    traverse(block_tree)                                       block.jl:127
+    ACA_plus(n_rows, n_cols, calc_rows, calc_cols, eps; max_iter)  aca.jl:36
+    SVD_recompress(U, V, eps)                                  aca.jl:123
+Coordinates are a (3, N) array (one column per point) as in Julia; indices in
+index maps and node ranges are 1-based, as the reference stores them.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _points_for_abi(coordinates):
+    """(3, N) coordinates -> contiguous (N, 3) buffer == 3xN column-major."""
+    c = np.asarray(coordinates, dtype=np.float64)
+    if c.ndim != 2 or c.shape[0] != 3:
+        raise ValueError("coordinates must have shape (3, N), one column per point")
+    return np.ascontiguousarray(c.T)
+
+
+class ClusterNode:
+    """tree.jl:6-14 (start_idx 1-based, end_idx exclusive)."""
+
+    __slots__ = ("is_leaf", "start_idx", "end_idx", "center", "radius", "left", "right", "id")
+
+    def __repr__(self):
+        return "ClusterNode(id=%d, [%d, %d), leaf=%s)" % (self.id, self.start_idx, self.end_idx,
+                                                          self.is_leaf)
+
+
+class ClusterTree:
+    """Geometric cluster tree (tree.jl:17-67), built natively by libhmx."""
+
+    def __init__(self, coordinates, max_points_per_leaf=32, dims=1):
+        pts = _points_for_abi(coordinates)
+        self.coordinates = pts
+        h = L.vp()
+        L.check(L.lib.hmx_tree_build(L.ptr_f64(pts), pts.shape[0], int(max_points_per_leaf),
+                                     int(dims), C.byref(h)))
+        self._h = h
+        ni, nn = C.c_int64(), C.c_int64()
+        L.check(L.lib.hmx_tree_sizes(h, C.byref(ni), C.byref(nn)))
+        self.index_map = np.zeros(ni.value, np.int64)
+        self.node_start = np.zeros(nn.value, np.int64)
+        self.node_end = np.zeros(nn.value, np.int64)
+        self.node_center = np.zeros((nn.value, 3))
+        self.node_radius = np.zeros(nn.value)
+        self.node_left = np.zeros(nn.value, np.int64)
+        self.node_right = np.zeros(nn.value, np.int64)
+        self.node_is_leaf = np.zeros(nn.value, np.int8)
+        L.check(L.lib.hmx_tree_export(h, L.ptr_i64(self.index_map), L.ptr_i64(self.node_start),
+                                      L.ptr_i64(self.node_end), L.ptr_f64(self.node_center),
+                                      L.ptr_f64(self.node_radius), L.ptr_i64(self.node_left),
+                                      L.ptr_i64(self.node_right), L.ptr_i8(self.node_is_leaf)))
+        self.dims = int(dims)
+        self._nodes = {}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def node(self, k):
+        """ClusterNode view of node id k (pre-order numbering, root = 0)."""
+        k = int(k)
+        if k < 0:
+            return None
+        nd = self._nodes.get(k)
+        if nd is None:
+            nd = ClusterNode()
+            nd.id = k
+            nd.is_leaf = bool(self.node_is_leaf[k])
+            nd.start_idx = int(self.node_start[k])
+            nd.end_idx = int(self.node_end[k])
+            nd.center = self.node_center[k].copy()
+            nd.radius = float(self.node_radius[k])
+            nd.left = None
+            nd.right = None
+            self._nodes[k] = nd
+            if self.node_left[k] >= 0:
+                nd.left = self.node(self.node_left[k])
+                nd.right = self.node(self.node_right[k])
+        return nd
+
+    @property
+    def root(self):
+        return self.node(0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib.hmx_tree_destroy(h)
+            self._h = None
+
+
+class BlockTreeNode:
+    """block.jl:2-9"""
+
+    __slots__ = ("target_tree", "source_tree", "is_admissible", "is_dense", "left", "right",
+                 "id", "_bt")
+
+
+class BlockTree:
+    """Block cluster tree (block.jl:12-44)."""
+
+    def __init__(self, X, Y, eta=1.5, index_map_using_cpu=True):
+        self.X, self.Y = X, Y
+        h = L.vp()
+        L.check(L.lib.hmx_blocktree_build(X.handle, Y.handle, float(eta), C.byref(h)))
+        self._h = h
+        self.eta = float(eta)
+        self.target_index_map = X.index_map
+        self.source_index_map = Y.index_map
+        self._refresh()
+
+    def _refresh(self):
+        nn, nd, na = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(L.lib.hmx_blocktree_sizes(self._h, C.byref(nn), C.byref(nd), C.byref(na)))
+        n = nn.value
+        self.node_target = np.zeros(n, np.int64)
+        self.node_source = np.zeros(n, np.int64)
+        self.node_admissible = np.zeros(n, np.int8)
+        self.node_dense = np.zeros(n, np.int8)
+        self.node_left = np.zeros(n, np.int64)
+        self.node_right = np.zeros(n, np.int64)
+        L.check(L.lib.hmx_blocktree_export(self._h, L.ptr_i64(self.node_target),
+                                           L.ptr_i64(self.node_source),
+                                           L.ptr_i8(self.node_admissible),
+                                           L.ptr_i8(self.node_dense), L.ptr_i64(self.node_left),
+                                           L.ptr_i64(self.node_right)))
+        self._nd, self._na = nd.value, na.value
+        self._nodes = {}
+
+    def node(self, k):
+        k = int(k)
+        if k < 0:
+            return None
+        nd = self._nodes.get(k)
+        if nd is None:
+            nd = BlockTreeNode()
+            nd.id = k
+            nd._bt = self
+            nd.target_tree = self.X.node(self.node_target[k])
+            nd.source_tree = self.Y.node(self.node_source[k])
+            nd.is_admissible = bool(self.node_admissible[k])
+            nd.is_dense = bool(self.node_dense[k])
+            self._nodes[k] = nd
+            nd.left = self.node(self.node_left[k]) if self.node_left[k] >= 0 else None
+            nd.right = self.node(self.node_right[k]) if self.node_right[k] >= 0 else None
+        return nd
+
+    @property
+    def root(self):
+        return self.node(0)
+
+    def lists(self):
+        """traverse() as node-id arrays: (direct (nd,2), approx (na,2))."""
+        d = np.zeros((max(self._nd, 1), 2), np.int64)
+        a = np.zeros((max(self._na, 1), 2), np.int64)
+        L.check(L.lib.hmx_blocktree_traverse(self._h, L.ptr_i64(d), L.ptr_i64(a)))
+        return d[:self._nd], a[:self._na]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib.hmx_blocktree_destroy(h)
+            self._h = None
+
+
+def merge_dense_matrices_(node):
+    """merge_dense_matrices! (block.jl:98-112); accepts the block tree or its root."""
+    bt = node if isinstance(node, BlockTree) else node._bt
+    L.check(L.lib.hmx_blocktree_merge_dense(bt._h))
+    bt._refresh()
+
+
+def traverse(block_tree):
+    """traverse (block.jl:127-155): (direct_list, approx_list) of
+    (target ClusterNode, source ClusterNode) tuples in DFS order."""
+    d, a = block_tree.lists()
+    X, Y = block_tree.X, block_tree.Y
+    return ([(X.node(t), Y.node(s)) for t, s in d], [(X.node(t), Y.node(s)) for t, s in a])
+
+
+def info(obj):
+    """info(::ClusterTree) (tree.jl:153-187) or info(::HMatrix) (hmatrix.jl:289-330)."""
+    if isinstance(obj, ClusterTree):
+        d, mx, mn = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(L.lib.hmx_tree_info(obj.handle, C.byref(d), C.byref(mx), C.byref(mn)))
+        return {"depth": d.value, "max_points": mx.value, "min_points": mn.value}
+    return obj.info()
+
+
+class _LowRank:
+    def __init__(self, h):
+        self._h = h
+
+    def arrays(self):
+        m, n, k = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(L.lib.hmx_lowrank_sizes(self._h, C.byref(m), C.byref(n), C.byref(k)))
+        U = np.zeros((m.value, k.value), order="F")
+        V = np.zeros((k.value, n.value), order="F")
+        L.check(L.lib.hmx_lowrank_export(self._h, U.ctypes.data_as(L.f64p),
+                                         V.ctypes.data_as(L.f64p)))
+        return U, V
+
+    def __del__(self):
+        if self._h is not None and self._h.value:
+            L.lib.hmx_lowrank_destroy(self._h)
+
+
+def ACA_plus(n_rows, n_cols, calc_rows, calc_cols, eps, max_iter=None):
+    """ACA+ (aca.jl:36-106).  calc_rows(I) / calc_cols(J) take 1-based indices
+    and return the full row / column.  Returns (U, V) with U n_rows x k."""
+    err = []
+
+    def row_cb(ctx, i, out):
+        try:
+            out_arr = np.ctypeslib.as_array(out, shape=(n_cols,))
+            out_arr[:] = np.asarray(calc_rows(int(i)), dtype=np.float64).reshape(-1)
+            return 0
+        except Exception as e:  # noqa: BLE001 - reported after the call
+            err.append(e)
+            return 1
+
+    def col_cb(ctx, j, out):
+        try:
+            out_arr = np.ctypeslib.as_array(out, shape=(n_rows,))
+            out_arr[:] = np.asarray(calc_cols(int(j)), dtype=np.float64).reshape(-1)
+            return 0
+        except Exception as e:  # noqa: BLE001
+            err.append(e)
+            return 1
+
+    rf, cf = L.VEC_FN(row_cb), L.VEC_FN(col_cb)
+    h = L.vp()
+    st = L.lib.hmx_aca_plus(int(n_rows), int(n_cols), rf, cf, None, float(eps),
+                            int(max_iter or 0), C.byref(h))
+    if err:
+        raise err[0]
+    L.check(st)
+    return _LowRank(h).arrays()
+
+
+def SVD_recompress(U, V, eps):
+    """SVD_recompress (aca.jl:123-143)."""
+    U = np.asfortranarray(U, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.float64)
+    m, k = U.shape
+    n = V.shape[1]
+    h = L.vp()
+    L.check(L.lib.hmx_lowrank_create(m, n, k, U.ctypes.data_as(L.f64p), V.ctypes.data_as(L.f64p),
+                                     C.byref(h)))
+    src = _LowRank(h)
+    h2 = L.vp()
+    L.check(L.lib.hmx_svd_recompress(src._h, float(eps), C.byref(h2)))
+    return _LowRank(h2).arrays()

@@ -1,0 +1,169 @@
+/*
+ * hmx.h -- C ABI of the MI355X (gfx950) H-matrix engine (libhmx.so).
+ *
+ * Every entry point mirrors a HMatrixGPU.jl v0.1.0 interface (cited per
+ * function) so that the Julia package (ccall), the Python mirror (ctypes) and
+ * C/C++ callers bind the same symbols.  Conventions:
+ *   - every function returns an int status, 0 == HMX_OK; on failure
+ *     hmx_last_error() returns a thread-local message;
+ *   - indices crossing the ABI are int64 and 1-based wherever the reference
+ *     stores them 1-based (index maps, node ranges, block tables);
+ *   - point coordinates are a 3 x N column-major double array (the Julia
+ *     Matrix{Float64} the reference takes);
+ *   - device pointers are plain addresses on the current HIP device; streams
+ *     are hipStream_t passed as void*.
+ */
+#ifndef HMX_H
+#define HMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status, types ------------------------------------------------------ */
+enum hmx_status {
+    HMX_OK = 0,
+    HMX_ERR_ARG = 1,           /* bad argument (NULL handle, size mismatch, ...)   */
+    HMX_ERR_DIMS = 2,          /* dims <= 0, tree.jl:46-48                         */
+    HMX_ERR_EMPTY_CLUSTER = 3, /* a split left an empty cluster (the reference
+                                  throws reducing over an empty point set)        */
+    HMX_ERR_NOMEM = 4,
+    HMX_ERR_HIP = 5,           /* HIP runtime error / no device                    */
+    HMX_ERR_CALLBACK = 6,      /* user entry callback returned non-zero            */
+    HMX_ERR_STATE = 7,         /* object not in the required state                 */
+    HMX_ERR_UNSUPPORTED = 8
+};
+
+enum hmx_dtype { HMX_F64 = 0, HMX_F32 = 1, HMX_BF16 = 2 };
+
+/* Built-in kernels evaluated natively on host and device.                     */
+enum hmx_kernel_id {
+    HMX_KERNEL_CALLBACK = 0, /* entries come from an hmx_block_fn               */
+    HMX_KERNEL_LAPLACE = 1,  /* 1/|x-y|; 0 where points coincide (the diagonal)  */
+    HMX_KERNEL_LOG2D = 2     /* -log(d)/(2*pi); 1.0 where d < 1e-12              */
+};
+
+typedef struct hmx_tree hmx_tree;
+typedef struct hmx_blocktree hmx_blocktree;
+typedef struct hmx_lowrank hmx_lowrank;
+typedef struct hmx_hmatrix hmx_hmatrix;
+
+/* Matrix entries for HMX_KERNEL_CALLBACK: fill out[i + j*nrows] = K[rows[i], cols[j]]
+ * (1-based original indices, column-major), i.e. Julia's K[rows, cols]
+ * getindex contract used at hmatrix.jl:183,195-196,209.  Return 0 on success. */
+typedef int (*hmx_block_fn)(void* ctx, const int64_t* rows, int64_t nrows, const int64_t* cols,
+                            int64_t ncols, double* out);
+
+/* One row (or column) of the matrix given to ACA_plus: out has n_cols (n_rows)
+ * entries; index is 1-based.  Mirrors calc_rows / calc_cols, aca.jl:36. */
+typedef int (*hmx_vec_fn)(void* ctx, int64_t index, double* out);
+
+const char* hmx_last_error(void);
+int hmx_version(int32_t* major, int32_t* minor, int32_t* patch);
+
+/* ---- cluster tree: replaces ClusterTree(...) (tree.jl:45-67, build_tree_node
+ *      tree.jl:70-116, update_tree_indices! tree.jl:123-137) ------------------ */
+int hmx_tree_build(const double* xyz, int64_t npoints, int32_t max_points_per_leaf, int32_t dims,
+                   hmx_tree** out);
+/* number of index_map entries (npoints*dims) and of nodes (pre-order) */
+int hmx_tree_sizes(const hmx_tree* t, int64_t* nindex, int64_t* nnodes);
+/* ClusterTree fields (tree.jl:6-20): index_map (1-based), per node start_idx,
+ * end_idx (exclusive), center (3 per node), radius, left/right child node ids
+ * (-1 = none), is_leaf.  Any output pointer may be NULL. */
+int hmx_tree_export(const hmx_tree* t, int64_t* index_map, int64_t* start_idx, int64_t* end_idx,
+                    double* center, double* radius, int64_t* left, int64_t* right,
+                    int8_t* is_leaf);
+/* info(::ClusterTree), tree.jl:153-187 */
+int hmx_tree_info(const hmx_tree* t, int64_t* depth, int64_t* max_points, int64_t* min_points);
+int hmx_tree_destroy(hmx_tree* t);
+
+/* ---- block tree: replaces BlockTree(X, Y; eta) (block.jl:31-44,58-90),
+ *      merge_dense_matrices! (block.jl:98-112) and traverse (block.jl:127-155) - */
+int hmx_blocktree_build(const hmx_tree* target, const hmx_tree* source, double eta,
+                        hmx_blocktree** out);
+int hmx_blocktree_merge_dense(hmx_blocktree* b);
+/* nnodes, and the lengths of direct_list / approx_list traverse() returns */
+int hmx_blocktree_sizes(const hmx_blocktree* b, int64_t* nnodes, int64_t* ndirect,
+                        int64_t* napprox);
+/* BlockTreeNode fields (block.jl:2-9): target/source cluster node ids,
+ * is_admissible, is_dense, left/right block node ids (-1 = none). */
+int hmx_blocktree_export(const hmx_blocktree* b, int64_t* target_node, int64_t* source_node,
+                         int8_t* is_admissible, int8_t* is_dense, int64_t* left, int64_t* right);
+/* traverse(): (target node, source node) pairs, 2 x n each, DFS order */
+int hmx_blocktree_traverse(const hmx_blocktree* b, int64_t* direct, int64_t* approx);
+int hmx_blocktree_destroy(hmx_blocktree* b);
+
+/* ---- low-rank compression: replaces ACA_plus (aca.jl:36-106) and
+ *      SVD_recompress (aca.jl:123-143) ---------------------------------------- */
+/* max_iter <= 0 means the reference default (n_rows). */
+int hmx_aca_plus(int64_t n_rows, int64_t n_cols, hmx_vec_fn calc_rows, hmx_vec_fn calc_cols,
+                 void* ctx, double eps, int64_t max_iter, hmx_lowrank** out);
+int hmx_lowrank_create(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
+                       hmx_lowrank** out);
+int hmx_lowrank_sizes(const hmx_lowrank* lr, int64_t* m, int64_t* n, int64_t* rank);
+/* U: m x k column-major, V: k x n column-major */
+int hmx_lowrank_export(const hmx_lowrank* lr, double* U, double* V);
+int hmx_svd_recompress(const hmx_lowrank* in, double eps, hmx_lowrank** out);
+int hmx_lowrank_destroy(hmx_lowrank* lr);
+
+/* ---- H-matrix: replaces HMatrix(K, X, Y; eta, eps, flatten, ...,
+ *      svd_recompress) (hmatrix.jl:72-148, build_matrices hmatrix.jl:170-217) --
+ * kernel_id selects a built-in kernel (target_xyz / source_xyz: 3 x N column-
+ * major, the same arrays the trees were built from) or HMX_KERNEL_CALLBACK
+ * (fill/ctx).  storage_mask: OR of (1 << HMX_F64 | HMX_F32 | HMX_BF16), the
+ * storage precisions to materialise on `device` (0 = host-only build; the
+ * reference layout can still be exported).  nthreads <= 0: library default.  */
+int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                      const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
+                      void* ctx, double eta, double eps, int32_t svd_recompress,
+                      int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out);
+/* sizes[0..7] = m, n, n_dense, n_lowrank, R (= length(Vx_buffer)), length of
+ * dense_blocks, U_matrices, V_matrices in the flattened layout. */
+int hmx_hmatrix_sizes(const hmx_hmatrix* h, int64_t* sizes);
+/* The reference's flattened layout (HMatrix{T} fields, hmatrix.jl:46-57,
+ * 95-147) exactly: index maps (1-based), dense_block_indices 5 x n_dense,
+ * U_block_indices 5 x n_lowrank (rows 3-4 = Vx range), V_block_indices 3 x R,
+ * row-major payloads.  Any pointer may be NULL. */
+int hmx_hmatrix_export_reference_layout(const hmx_hmatrix* h, int64_t* target_index_map,
+                                        int64_t* source_index_map, int64_t* dense_block_indices,
+                                        int64_t* U_block_indices, int64_t* V_block_indices,
+                                        double* dense_blocks, double* U_matrices,
+                                        double* V_matrices);
+/* HMatrixCPU's dense_block_indices / approx_block_indices (hmatrix.jl:18-27):
+ * (row_start, row_end, col_start, col_end), 1-based inclusive, 4 x n each, in
+ * list order (dense: direct list then storage fallbacks). */
+int hmx_hmatrix_export_block_ranges(const hmx_hmatrix* h, int64_t* dense_ranges,
+                                    int64_t* lowrank_ranges);
+/* info(::HMatrixCPU) (hmatrix.jl:289-330) plus byte counts:
+ * out[0] leaves, [1] admissible_leaves, [2] full_leaves, [3] min_rank,
+ * [4] max_rank, [5] min_dense_size, [6] max_dense_size, [7] compression_ratio,
+ * [8] nnz_D, [9] nnz_U, [10] nnz_V, [11] device panel bytes F64, [12] F32,
+ * [13] BF16, [14] device index/plan bytes (all storages), [15] build seconds */
+int hmx_hmatrix_info(const hmx_hmatrix* h, double* out);
+int hmx_hmatrix_destroy(hmx_hmatrix* h);
+
+/* ---- product: replaces mul!(result, H::HMatrix{T}, x) (mult.jl:159-189) ----
+ * Y[:, c] = H * X[:, c] for c < nrhs.  X (n x nrhs, leading dim ldx) and Y
+ * (m x nrhs, ldy) are device pointers of io_dtype (HMX_F64 or HMX_F32) on the
+ * H-matrix's device; `storage` selects which materialised precision is used.
+ * Y is fully overwritten (no zeroing needed); X is read-only.  Asynchronous
+ * and stream-ordered on `stream` (NULL = default stream).  Scratch is per
+ * (H-matrix, stream), so concurrent calls on different streams are safe;
+ * concurrent calls on one stream are ordered by the stream.  The first call on
+ * a new stream allocates its scratch (hipMalloc). */
+int hmx_matvec(hmx_hmatrix* h, int32_t storage, const void* X, int64_t ldx, void* Y, int64_t ldy,
+               int32_t nrhs, int32_t io_dtype, void* stream);
+
+/* Per-stage kernel timing with HIP events recorded on the launch stream
+ * (stage 0 gather, 1 far-field V*X, 2 partial reduce, 3 near+far U panel
+ * product).  enable != 0 starts recording; read returns summed milliseconds
+ * and launch counts per stage after synchronising the recorded events. */
+int hmx_profile_enable(hmx_hmatrix* h, int32_t enable);
+int hmx_profile_read(hmx_hmatrix* h, double* ms, int64_t* launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

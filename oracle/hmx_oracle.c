@@ -1,0 +1,995 @@
+/*
+ * hmx_oracle.c -- TEST INFRASTRUCTURE ONLY.  A plain-C FP64 restatement of the
+ * reference HMatrixGPU.jl v0.1.0 (src/tree.jl, src/block.jl, src/aca.jl,
+ * src/hmatrix.jl, src/mult.jl).  The product library never links or loads it;
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it as the
+ * checker.  Compile with -ffp-contract=off (see oracle/Makefile).
+ *
+ * Pinning (what fixes each function independently of this file's own formulas;
+ * the tests live in tests/test_oracle_*.py):
+ *   orc_norm3            closed forms, under/overflow scaling branch
+ *   orc_tree_build       hand-computed splits, brute-force recomputation of every
+ *                        node's bbox/centroid/radius, stability, leaf bounds,
+ *                        dims expansion (test_tree.jl, test_hmatrix_vector.jl:33-35)
+ *   orc_btree_*          admissibility invariant, childless labels, exact
+ *                        partition of unity by brute-force coverage
+ *                        (test_block.jl:19-97), merge/DFS order on hand-built cases
+ *   orc_aca_plus         known-answer N/(i-j)^2 block (test_aca.jl:7-31), exact
+ *                        low-rank inputs, brute-force argmax tie-breaks, hand
+ *                        traced pivot sequence
+ *   orc_qr/orc_jacobi_svd/orc_svd_recompress
+ *                        scipy/LAPACK (geqrf/gesdd) singular values, Eckart-Young
+ *                        residual identity, truncation rule on built spectra
+ *   orc_hmatrix_build    layout invariants (test_hmatrix.jl:52-55), compression
+ *                        ratio and accuracy vs the dense product
+ *                        (test_hmatrix.jl:30-50), storage-fallback ordering
+ *   orc_mul_layout       dense direct sum (orc_dense_matvec) within the H-matrix
+ *                        tolerance, and numpy evaluation of the exported layout
+ * Parity unpinned: bitwise agreement with Julia's SIMD-reassociated sum in the
+ * ACA stop test (aca.jl:95) and with LAPACK's QR/SVD bits (aca.jl:125-129);
+ * only threshold-borderline rank decisions can differ.
+ */
+#include "hmx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+void orc_free(void* p) { free(p); }
+
+/* ------------------------------------------------------------------------ */
+/* Julia 1.10 Float64 min/max (Base.min/max for IEEEFloat, branch-free form)  */
+static double jl_min(double x, double y) {
+    double diff = x - y;
+    double r = signbit(diff) ? x : y;
+    return (isnan(x) || isnan(y)) ? diff : r;
+}
+static double jl_max(double x, double y) {
+    double diff = x - y;
+    double r = signbit(diff) ? y : x;
+    return (isnan(x) || isnan(y)) ? diff : r;
+}
+/* isless(a,b) for Float64: NaN sorts last (largest) */
+static int jl_isless(double a, double b) {
+    if (isnan(a)) return 0;
+    if (isnan(b)) return 1;
+    if (a < b) return 1;
+    if (a == b) return signbit(a) && !signbit(b);
+    return 0;
+}
+
+/* LinearAlgebra.generic_norm2 on a 3-element array (NRM2_CUTOFF=32 keeps it
+ * off BLAS): tree.jl:86-87, block.jl:63. */
+double orc_norm3(const double v[3]) {
+    double maxabs = jl_max(jl_max(fabs(v[0]), fabs(v[1])), fabs(v[2]));
+    if (maxabs == 0.0 || isinf(maxabs)) return maxabs;
+    if (isfinite(3.0 * maxabs * maxabs) && (maxabs * maxabs) != 0.0) {
+        double s = v[0] * v[0];
+        s += v[1] * v[1];
+        s += v[2] * v[2];
+        return sqrt(s);
+    }
+    double a = fabs(v[0]) / maxabs;
+    double s = a * a;
+    a = fabs(v[1]) / maxabs;
+    s += a * a;
+    a = fabs(v[2]) / maxabs;
+    s += a * a;
+    return maxabs * sqrt(s);
+}
+
+double orc_kernel_entry(const orc_kernel* K, int64_t i, int64_t j) {
+    switch (K->kind) {
+    case ORC_K_LAPLACE: {
+        const double* a = K->xt + 3 * i;
+        const double* b = K->xs + 3 * j;
+        double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        double d2 = dx * dx + dy * dy;
+        d2 = d2 + dz * dz;
+        double d = sqrt(d2);
+        return d == 0.0 ? 0.0 : 1.0 / d;
+    }
+    case ORC_K_LOG2D: {
+        const double* a = K->xt + 3 * i;
+        const double* b = K->xs + 3 * j;
+        double r[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+        double d = orc_norm3(r);
+        return d < 1e-12 ? 1.0 : (-0.5 / 3.141592653589793) * log(d);
+    }
+    case ORC_K_MATRIX:
+        return K->mat[i + j * K->ld];
+    case ORC_K_VEC3: {
+        int64_t J = j / 3, c = j % 3;
+        const double* a = K->xt + 3 * i;
+        const double* b = K->xs + 3 * J;
+        double r[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+        double d = orc_norm3(r);
+        return d < 1e-12 ? 1.0 : r[c] / (d * d);
+    }
+    default:
+        return NAN;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cluster tree, tree.jl:45-137                                               */
+typedef struct {
+    int64_t start, end; /* Julia fields: 1-based start, exclusive end */
+    double center[3];
+    double radius;
+    int64_t left, right;
+    int8_t is_leaf;
+} orc_node;
+
+struct orc_tree {
+    int64_t npts, dims, nidx;
+    int64_t* index_map; /* 1-based values, length nidx */
+    orc_node* nodes;
+    int64_t nnodes, cap;
+};
+
+static int64_t tree_push(orc_tree* t) {
+    if (t->nnodes == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 64;
+        t->nodes = (orc_node*)realloc(t->nodes, (size_t)t->cap * sizeof(orc_node));
+    }
+    memset(&t->nodes[t->nnodes], 0, sizeof(orc_node));
+    return t->nnodes++;
+}
+
+/* build_tree_node, tree.jl:70-116.  imap holds 1-based point ids. */
+static int64_t build_node(orc_tree* t, const double* xyz, int64_t leaf, int64_t* imap,
+                          int64_t start, int64_t end, int64_t* scratch, int32_t* err) {
+    int64_t n = end - start;
+    if (n <= 0) { /* minimum over an empty point set throws in the reference */
+        *err = 2;
+        return -1;
+    }
+    int64_t* v = imap + (start - 1);
+    double mn[3], mx[3], sum[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; d++) mn[d] = mx[d] = xyz[3 * (v[0] - 1) + d];
+    for (int64_t p = 0; p < n; p++) {
+        const double* q = xyz + 3 * (v[p] - 1);
+        for (int d = 0; d < 3; d++) {
+            mn[d] = jl_min(mn[d], q[d]);
+            mx[d] = jl_max(mx[d], q[d]);
+            sum[d] = sum[d] + q[d]; /* sequential sum over points, tree.jl:85 */
+        }
+    }
+    double len[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    int axis = 0; /* argmax: first maximal element */
+    for (int d = 1; d < 3; d++)
+        if (jl_isless(len[axis], len[d])) axis = d;
+    double c[3];
+    for (int d = 0; d < 3; d++) c[d] = sum[d] / (double)n;
+    double dmax[3] = {c[0] - mx[0], c[1] - mx[1], c[2] - mx[2]};
+    double dmin[3] = {c[0] - mn[0], c[1] - mn[1], c[2] - mn[2]};
+    double radius = jl_max(orc_norm3(dmax), orc_norm3(dmin));
+
+    int64_t id = tree_push(t);
+    orc_node* nd = &t->nodes[id];
+    nd->start = start;
+    nd->end = end;
+    memcpy(nd->center, c, sizeof c);
+    nd->radius = radius;
+    nd->left = nd->right = -1;
+    if (n <= leaf) {
+        nd->is_leaf = 1;
+        return id;
+    }
+    /* stable partition on coord < center[axis] (strict), tree.jl:96-105 */
+    int64_t nl = 0, nr = 0;
+    double cut = c[axis];
+    for (int64_t p = 0; p < n; p++)
+        if (xyz[3 * (v[p] - 1) + axis] < cut) scratch[nl++] = v[p];
+    for (int64_t p = 0; p < n; p++)
+        if (!(xyz[3 * (v[p] - 1) + axis] < cut)) scratch[nl + nr++] = v[p];
+    memcpy(v, scratch, (size_t)n * sizeof(int64_t));
+    int64_t split = start + nl;
+    int64_t l = build_node(t, xyz, leaf, imap, start, split, scratch, err);
+    if (*err) return -1;
+    int64_t r = build_node(t, xyz, leaf, imap, split, end, scratch, err);
+    if (*err) return -1;
+    t->nodes[id].left = l;
+    t->nodes[id].right = r;
+    return id;
+}
+
+orc_tree* orc_tree_build(const double* xyz, int64_t npts, int64_t leaf, int64_t dims,
+                         int32_t* err) {
+    *err = 0;
+    if (dims <= 0) {
+        *err = 1;
+        return NULL;
+    }
+    if (npts <= 0 || !xyz) {
+        *err = 3;
+        return NULL;
+    }
+    orc_tree* t = (orc_tree*)calloc(1, sizeof(orc_tree));
+    t->npts = npts;
+    t->dims = dims;
+    int64_t* imap = (int64_t*)malloc((size_t)npts * sizeof(int64_t));
+    int64_t* scratch = (int64_t*)malloc((size_t)npts * sizeof(int64_t));
+    for (int64_t i = 0; i < npts; i++) imap[i] = i + 1;
+    build_node(t, xyz, leaf, imap, 1, npts + 1, scratch, err);
+    free(scratch);
+    if (*err) {
+        free(imap);
+        free(t->nodes);
+        free(t);
+        return NULL;
+    }
+    if (dims > 1) { /* tree.jl:59-64 and update_tree_indices!, :123-137 */
+        int64_t* e = (int64_t*)malloc((size_t)(npts * dims) * sizeof(int64_t));
+        for (int64_t p = 0; p < npts; p++)
+            for (int64_t i = 1; i <= dims; i++) e[p * dims + (i - 1)] = dims * imap[p] - (dims - i);
+        free(imap);
+        imap = e;
+        for (int64_t k = 0; k < t->nnodes; k++) {
+            orc_node* nd = &t->nodes[k];
+            int64_t s = nd->start;
+            nd->start = dims * (s - 1) + 1;
+            nd->end = dims * (nd->end - s) + nd->start;
+        }
+    }
+    t->index_map = imap;
+    t->nidx = npts * dims;
+    return t;
+}
+
+int64_t orc_tree_nnodes(const orc_tree* t) { return t->nnodes; }
+int64_t orc_tree_nindex(const orc_tree* t) { return t->nidx; }
+
+void orc_tree_export(const orc_tree* t, int64_t* index_map, int64_t* start, int64_t* end,
+                     double* center, double* radius, int64_t* left, int64_t* right,
+                     int8_t* is_leaf) {
+    if (index_map) memcpy(index_map, t->index_map, (size_t)t->nidx * sizeof(int64_t));
+    for (int64_t k = 0; k < t->nnodes; k++) {
+        const orc_node* nd = &t->nodes[k];
+        if (start) start[k] = nd->start;
+        if (end) end[k] = nd->end;
+        if (center) memcpy(center + 3 * k, nd->center, 3 * sizeof(double));
+        if (radius) radius[k] = nd->radius;
+        if (left) left[k] = nd->left;
+        if (right) right[k] = nd->right;
+        if (is_leaf) is_leaf[k] = nd->is_leaf;
+    }
+}
+
+static void info_rec(const orc_tree* t, int64_t id, int64_t depth, int64_t* dmax, int64_t* pmax,
+                     int64_t* pmin) {
+    const orc_node* nd = &t->nodes[id];
+    if (depth > *dmax) *dmax = depth;
+    if (nd->is_leaf) {
+        int64_t c = nd->end - nd->start;
+        if (c > *pmax) *pmax = c;
+        if (*pmin < 0 || c < *pmin) *pmin = c;
+    }
+    if (nd->left >= 0) info_rec(t, nd->left, depth + 1, dmax, pmax, pmin);
+    if (nd->right >= 0) info_rec(t, nd->right, depth + 1, dmax, pmax, pmin);
+}
+
+void orc_tree_info(const orc_tree* t, int64_t* depth, int64_t* max_points, int64_t* min_points) {
+    int64_t d = 0, pmax = 0, pmin = -1;
+    info_rec(t, 0, 1, &d, &pmax, &pmin);
+    *depth = d;
+    *max_points = pmax;
+    *min_points = pmin < 0 ? 0 : pmin;
+}
+
+void orc_tree_free(orc_tree* t) {
+    if (!t) return;
+    free(t->index_map);
+    free(t->nodes);
+    free(t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block tree, block.jl:31-155                                                */
+typedef struct {
+    int64_t t, s;
+    int8_t adm, dense;
+    int64_t left, right;
+} orc_bnode;
+
+struct orc_btree {
+    const orc_tree *T, *S;
+    orc_bnode* nodes;
+    int64_t nnodes, cap;
+};
+
+static int64_t bt_push(orc_btree* b, int64_t t, int64_t s) {
+    if (b->nnodes == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 256;
+        b->nodes = (orc_bnode*)realloc(b->nodes, (size_t)b->cap * sizeof(orc_bnode));
+    }
+    orc_bnode* n = &b->nodes[b->nnodes];
+    n->t = t;
+    n->s = s;
+    n->adm = n->dense = 0;
+    n->left = n->right = -1;
+    return b->nnodes++;
+}
+
+/* build_block_tree!, block.jl:58-90 */
+static void bt_build(orc_btree* b, int64_t id, double eta) {
+    const orc_node* X = &b->T->nodes[b->nodes[id].t];
+    const orc_node* Y = &b->S->nodes[b->nodes[id].s];
+    double d[3] = {X->center[0] - Y->center[0], X->center[1] - Y->center[1],
+                   X->center[2] - Y->center[2]};
+    double dist = orc_norm3(d);
+    if (dist > eta * (X->radius + Y->radius)) {
+        b->nodes[id].adm = 1;
+        return;
+    }
+    if (X->is_leaf && Y->is_leaf) {
+        b->nodes[id].dense = 1;
+        return;
+    }
+    int split_Y = (!Y->is_leaf && Y->radius > X->radius) || X->is_leaf;
+    int64_t l, r;
+    int64_t tid = b->nodes[id].t, sid = b->nodes[id].s;
+    if (split_Y) {
+        l = bt_push(b, tid, Y->left);
+        r = bt_push(b, tid, Y->right);
+    } else {
+        l = bt_push(b, X->left, sid);
+        r = bt_push(b, X->right, sid);
+    }
+    b->nodes[id].left = l;
+    b->nodes[id].right = r;
+    bt_build(b, l, eta);
+    bt_build(b, r, eta);
+}
+
+orc_btree* orc_btree_build(const orc_tree* target, const orc_tree* source, double eta) {
+    orc_btree* b = (orc_btree*)calloc(1, sizeof(orc_btree));
+    b->T = target;
+    b->S = source;
+    bt_push(b, 0, 0);
+    bt_build(b, 0, eta);
+    return b;
+}
+
+/* merge_dense_matrices!, block.jl:98-112 */
+static void bt_merge(orc_btree* b, int64_t id) {
+    orc_bnode* n = &b->nodes[id];
+    if (n->left >= 0 && n->right >= 0) {
+        bt_merge(b, n->left);
+        bt_merge(b, n->right);
+        n = &b->nodes[id];
+        n->dense = b->nodes[n->left].dense && b->nodes[n->right].dense;
+    }
+}
+void orc_btree_merge(orc_btree* b) { bt_merge(b, 0); }
+
+int64_t orc_btree_nnodes(const orc_btree* b) { return b->nnodes; }
+
+void orc_btree_export(const orc_btree* b, int64_t* tnode, int64_t* snode, int8_t* adm,
+                      int8_t* dense, int64_t* left, int64_t* right) {
+    for (int64_t k = 0; k < b->nnodes; k++) {
+        const orc_bnode* n = &b->nodes[k];
+        tnode[k] = n->t;
+        snode[k] = n->s;
+        adm[k] = n->adm;
+        dense[k] = n->dense;
+        left[k] = n->left;
+        right[k] = n->right;
+    }
+}
+
+/* traverse, block.jl:127-155 (DFS pre-order, left before right) */
+static void bt_trav(const orc_btree* b, int64_t id, int64_t* nd, int64_t* na, int64_t* direct,
+                    int64_t* approx) {
+    const orc_bnode* n = &b->nodes[id];
+    if (n->adm) {
+        if (approx) {
+            approx[2 * *na] = n->t;
+            approx[2 * *na + 1] = n->s;
+        }
+        (*na)++;
+    } else if (n->dense) {
+        if (direct) {
+            direct[2 * *nd] = n->t;
+            direct[2 * *nd + 1] = n->s;
+        }
+        (*nd)++;
+    }
+    if (!n->dense && !n->adm) {
+        if (n->left >= 0) bt_trav(b, n->left, nd, na, direct, approx);
+        if (n->right >= 0) bt_trav(b, n->right, nd, na, direct, approx);
+    }
+}
+
+void orc_btree_traverse_counts(const orc_btree* b, int64_t* ndirect, int64_t* napprox) {
+    *ndirect = *napprox = 0;
+    bt_trav(b, 0, ndirect, napprox, NULL, NULL);
+}
+void orc_btree_traverse(const orc_btree* b, int64_t* direct, int64_t* approx) {
+    int64_t nd = 0, na = 0;
+    bt_trav(b, 0, &nd, &na, direct, approx);
+}
+void orc_btree_free(orc_btree* b) {
+    if (!b) return;
+    free(b->nodes);
+    free(b);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ACA+, aca.jl:9-106                                                         */
+
+/* argmax_not_in_list (aca.jl:9-17): first index of the stable descending
+ * sortperm that is not used == largest value under isless, lowest index on
+ * ties.  Returns -1 when everything is used. */
+static int64_t argmax_allowed(const double* a, int64_t n, const char* used) {
+    int64_t best = -1;
+    for (int64_t i = 0; i < n; i++) {
+        if (used[i]) continue;
+        if (best < 0 || jl_isless(a[best], a[i])) best = i;
+    }
+    return best;
+}
+
+int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const int64_t* cols,
+                     int64_t n, double eps, int64_t max_iter, double** Uout, double** Vout) {
+    if (max_iter <= 0) max_iter = m; /* default max_iter = n_rows */
+    int64_t kmax = m < n ? m : n;
+    if (max_iter < kmax) kmax = max_iter; /* aca.jl:60 */
+    int64_t cap = kmax < 16 ? kmax : 16; /* vectors grow as terms are pushed */
+    double* left = (double*)malloc((size_t)(m * cap) * sizeof(double));
+    double* right = (double*)malloc((size_t)(n * cap) * sizeof(double));
+    char* used_r = (char*)calloc((size_t)m, 1);
+    char* used_c = (char*)calloc((size_t)n, 1);
+    double* rrow = (double*)calloc((size_t)n, sizeof(double));
+    double* rcol = (double*)calloc((size_t)m, sizeof(double));
+    double* arow = (double*)malloc((size_t)n * sizeof(double));
+    double* acol = (double*)malloc((size_t)m * sizeof(double));
+    int64_t k = 0;
+
+    for (int64_t it = 0; it < kmax; it++) {
+        for (int64_t j = 0; j < n; j++) arow[j] = fabs(rrow[j]);
+        for (int64_t i = 0; i < m; i++) acol[i] = fabs(rcol[i]);
+        int64_t rp, cp;
+        if (it == 0) {
+            rp = 0;
+            cp = 0;
+        } else {
+            cp = argmax_allowed(arow, n, used_c);
+            rp = argmax_allowed(acol, m, used_r);
+        }
+        double cval = arow[cp], rval = acol[rp];
+        /* calc_residual_rows / calc_residual_cols, aca.jl:43-57 */
+#define RES_ROW(I)                                                                  \
+    do {                                                                            \
+        for (int64_t j = 0; j < n; j++) rrow[j] = orc_kernel_entry(K, rows[I], cols[j]); \
+        for (int64_t q = 0; q < k; q++) {                                           \
+            double uq = left[q * m + (I)];                                          \
+            const double* vq = right + q * n;                                       \
+            for (int64_t j = 0; j < n; j++) rrow[j] = rrow[j] - uq * vq[j];         \
+        }                                                                           \
+    } while (0)
+#define RES_COL(J)                                                                  \
+    do {                                                                            \
+        for (int64_t i = 0; i < m; i++) rcol[i] = orc_kernel_entry(K, rows[i], cols[J]); \
+        for (int64_t q = 0; q < k; q++) {                                           \
+            double vq = right[q * n + (J)];                                         \
+            const double* uq = left + q * m;                                        \
+            for (int64_t i = 0; i < m; i++) rcol[i] = rcol[i] - uq[i] * vq;         \
+        }                                                                           \
+    } while (0)
+        if (rval > cval) {
+            RES_ROW(rp);
+            for (int64_t j = 0; j < n; j++) arow[j] = fabs(rrow[j]);
+            cp = argmax_allowed(arow, n, used_c);
+            RES_COL(cp);
+        } else {
+            RES_COL(cp);
+            for (int64_t i = 0; i < m; i++) acol[i] = fabs(rcol[i]);
+            rp = argmax_allowed(acol, m, used_r);
+            RES_ROW(rp);
+        }
+#undef RES_ROW
+#undef RES_COL
+        used_r[rp] = 1;
+        used_c[cp] = 1;
+        if (k == cap) {
+            cap = 2 * cap < kmax ? 2 * cap : kmax;
+            left = (double*)realloc(left, (size_t)(m * cap) * sizeof(double));
+            right = (double*)realloc(right, (size_t)(n * cap) * sizeof(double));
+        }
+        double piv = rrow[cp];
+        double* v = right + k * n;
+        double* u = left + k * m;
+        for (int64_t j = 0; j < n; j++) v[j] = rrow[j] / piv; /* aca.jl:92 */
+        memcpy(u, rcol, (size_t)m * sizeof(double));            /* aca.jl:93 */
+        k++;
+        double su = 0.0, sv = 0.0; /* aca.jl:95 (sequential order) */
+        for (int64_t i = 0; i < m; i++) su += u[i] * u[i];
+        for (int64_t j = 0; j < n; j++) sv += v[j] * v[j];
+        double step = sqrt(su * sv);
+        if (step < eps) break;
+    }
+    /* U = hcat(left...) (m x k), V = hcat(right...)' (k x n), column-major */
+    double* U = (double*)malloc((size_t)(m * (k ? k : 1)) * sizeof(double));
+    double* V = (double*)malloc((size_t)(n * (k ? k : 1)) * sizeof(double));
+    memcpy(U, left, (size_t)(m * k) * sizeof(double));
+    for (int64_t q = 0; q < k; q++)
+        for (int64_t j = 0; j < n; j++) V[q + j * k] = right[q * n + j];
+    free(left);
+    free(right);
+    free(used_r);
+    free(used_c);
+    free(rrow);
+    free(rcol);
+    free(arow);
+    free(acol);
+    *Uout = U;
+    *Vout = V;
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Householder QR (column-major m x k, m >= k).  R is left in the upper
+ * triangle, the Householder vectors (implicit unit head) below it. */
+void orc_qr(int64_t m, int64_t k, double* A, double* tau) {
+    for (int64_t j = 0; j < k; j++) {
+        double* a = A + j * m;
+        double alpha = a[j];
+        double xs = 0.0;
+        for (int64_t i = j + 1; i < m; i++) xs += a[i] * a[i];
+        if (xs == 0.0) {
+            tau[j] = 0.0;
+            continue;
+        }
+        double nrm = sqrt(alpha * alpha + xs);
+        double beta = alpha >= 0.0 ? -nrm : nrm;
+        tau[j] = (beta - alpha) / beta;
+        double scale = 1.0 / (alpha - beta);
+        for (int64_t i = j + 1; i < m; i++) a[i] = a[i] * scale;
+        a[j] = beta;
+        for (int64_t c = j + 1; c < k; c++) {
+            double* b = A + c * m;
+            double w = b[j];
+            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            w = w * tau[j];
+            b[j] = b[j] - w;
+            for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;
+        }
+    }
+}
+
+/* B <- Q B with Q = H_0 H_1 ... H_{k-1} (apply H_{k-1} first). */
+void orc_qr_apply(int64_t m, int64_t k, const double* A, const double* tau, int64_t ncols,
+                  double* B) {
+    for (int64_t j = k - 1; j >= 0; j--) {
+        if (tau[j] == 0.0) continue;
+        const double* a = A + j * m;
+        for (int64_t c = 0; c < ncols; c++) {
+            double* b = B + c * m;
+            double w = b[j];
+            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            w = w * tau[j];
+            b[j] = b[j] - w;
+            for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;
+        }
+    }
+}
+
+/* One-sided (Hestenes) Jacobi SVD of a k x k column-major M = W diag(sig) Z^T.
+ * Cyclic sweeps p<q, rotation skipped when |gamma| <= eps*sqrt(alpha*beta);
+ * singular values sorted descending (stable). */
+void orc_jacobi_svd(int64_t k, double* A, double* sig, double* Z) {
+    const double tol = 2.220446049250313e-16;
+    memset(Z, 0, (size_t)(k * k) * sizeof(double));
+    for (int64_t i = 0; i < k; i++) Z[i + i * k] = 1.0;
+    for (int sweep = 0; sweep < 80; sweep++) {
+        int rotated = 0;
+        for (int64_t p = 0; p + 1 < k; p++) {
+            for (int64_t q = p + 1; q < k; q++) {
+                double* ap = A + p * k;
+                double* aq = A + q * k;
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int64_t i = 0; i < k; i++) {
+                    al += ap[i] * ap[i];
+                    be += aq[i] * aq[i];
+                    ga += ap[i] * aq[i];
+                }
+                if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
+                rotated = 1;
+                double zeta = (be - al) / (2.0 * ga);
+                double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t);
+                double s = c * t;
+                for (int64_t i = 0; i < k; i++) {
+                    double x = ap[i], y = aq[i];
+                    ap[i] = c * x - s * y;
+                    aq[i] = s * x + c * y;
+                }
+                double* zp = Z + p * k;
+                double* zq = Z + q * k;
+                for (int64_t i = 0; i < k; i++) {
+                    double x = zp[i], y = zq[i];
+                    zp[i] = c * x - s * y;
+                    zq[i] = s * x + c * y;
+                }
+            }
+        }
+        if (!rotated) break;
+    }
+    double* s0 = (double*)malloc((size_t)k * sizeof(double));
+    int64_t* ord = (int64_t*)malloc((size_t)k * sizeof(int64_t));
+    for (int64_t j = 0; j < k; j++) {
+        double s = 0.0;
+        for (int64_t i = 0; i < k; i++) s += A[i + j * k] * A[i + j * k];
+        s0[j] = sqrt(s);
+        ord[j] = j;
+    }
+    for (int64_t a = 1; a < k; a++) { /* stable insertion sort, descending */
+        int64_t v = ord[a], b = a - 1;
+        while (b >= 0 && s0[ord[b]] < s0[v]) {
+            ord[b + 1] = ord[b];
+            b--;
+        }
+        ord[b + 1] = v;
+    }
+    double* W = (double*)malloc((size_t)(k * k) * sizeof(double));
+    double* Z2 = (double*)malloc((size_t)(k * k) * sizeof(double));
+    for (int64_t j = 0; j < k; j++) {
+        int64_t o = ord[j];
+        sig[j] = s0[o];
+        for (int64_t i = 0; i < k; i++) {
+            W[i + j * k] = s0[o] > 0.0 ? A[i + o * k] / s0[o] : 0.0;
+            Z2[i + j * k] = Z[i + o * k];
+        }
+    }
+    memcpy(A, W, (size_t)(k * k) * sizeof(double));
+    memcpy(Z, Z2, (size_t)(k * k) * sizeof(double));
+    free(W);
+    free(Z2);
+    free(s0);
+    free(ord);
+}
+
+/* SVD_recompress, aca.jl:123-143 */
+int64_t orc_svd_recompress(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
+                           double eps, double** U2out, double** V2out) {
+    double* QU = (double*)malloc((size_t)(m * k) * sizeof(double));
+    double* QV = (double*)malloc((size_t)(n * k) * sizeof(double));
+    double* tu = (double*)malloc((size_t)k * sizeof(double));
+    double* tv = (double*)malloc((size_t)k * sizeof(double));
+    memcpy(QU, U, (size_t)(m * k) * sizeof(double));
+    for (int64_t q = 0; q < k; q++) /* V' is n x k */
+        for (int64_t j = 0; j < n; j++) QV[j + q * n] = V[q + j * k];
+    orc_qr(m, k, QU, tu);
+    orc_qr(n, k, QV, tv);
+    double* M = (double*)malloc((size_t)(k * k) * sizeof(double));
+    for (int64_t i = 0; i < k; i++)
+        for (int64_t j = 0; j < k; j++) { /* (R_U R_V')[i,j] */
+            double s = 0.0;
+            for (int64_t l = (i > j ? i : j); l < k; l++) s += QU[i + l * m] * QV[j + l * n];
+            M[i + j * k] = s;
+        }
+    double* sig = (double*)malloc((size_t)k * sizeof(double));
+    double* Z = (double*)malloc((size_t)(k * k) * sizeof(double));
+    orc_jacobi_svd(k, M, sig, Z);
+    /* tail Frobenius norms: reverse(sqrt.(cumsum(reverse(SIG .^ 2)))) */
+    int64_t r = k;
+    {
+        double* fn = (double*)malloc((size_t)k * sizeof(double));
+        double acc = 0.0;
+        for (int64_t i = k - 1; i >= 0; i--) {
+            double s2 = sig[i] * sig[i];
+            acc = (i == k - 1) ? s2 : acc + s2;
+            fn[i] = sqrt(acc);
+        }
+        for (int64_t i = 0; i < k; i++)
+            if (fn[i] < eps) {
+                r = i;
+                break;
+            }
+        free(fn);
+    }
+    double* U2 = (double*)calloc((size_t)(m * (r ? r : 1)), sizeof(double));
+    for (int64_t j = 0; j < r; j++)
+        for (int64_t i = 0; i < k; i++) U2[i + j * m] = M[i + j * k] * sig[j];
+    orc_qr_apply(m, k, QU, tu, r, U2);
+    double* C = (double*)calloc((size_t)(n * (r ? r : 1)), sizeof(double));
+    for (int64_t j = 0; j < r; j++)
+        for (int64_t i = 0; i < k; i++) C[i + j * n] = Z[i + j * k];
+    orc_qr_apply(n, k, QV, tv, r, C);
+    double* V2 = (double*)malloc((size_t)(n * (r ? r : 1)) * sizeof(double));
+    for (int64_t j = 0; j < r; j++)
+        for (int64_t c = 0; c < n; c++) V2[j + c * r] = C[c + j * n];
+    free(C);
+    free(QU);
+    free(QV);
+    free(tu);
+    free(tv);
+    free(M);
+    free(sig);
+    free(Z);
+    *U2out = U2;
+    *V2out = V2;
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* H-matrix build + flatten, hmatrix.jl:72-217                                */
+struct orc_hmat {
+    int64_t m, n, nd, na, R;
+    int64_t *tmap, *smap;
+    int64_t *dense_idx, *U_idx, *V_idx;
+    double *D, *U, *V;
+    int64_t dlen, ulen, vlen;
+    int64_t* ranks;
+};
+
+typedef struct {
+    int64_t r0, r1, c0, c1; /* 1-based inclusive */
+    double* data;           /* row-major */
+} blk_dense;
+typedef struct {
+    int64_t r0, r1, c0, c1, k;
+    double *U, *V; /* U row-major m x k, V row-major k x n */
+} blk_lr;
+
+orc_hmat* orc_hmatrix_build(const orc_kernel* K, const orc_tree* T, const orc_tree* S, double eta,
+                            double eps, int32_t svd_recompress) {
+    orc_btree* bt = orc_btree_build(T, S, eta); /* hmatrix.jl:74 */
+    orc_btree_merge(bt);                        /* :75 */
+    int64_t nd0, na0;
+    orc_btree_traverse_counts(bt, &nd0, &na0); /* :78 */
+    int64_t* dl = (int64_t*)malloc((size_t)(2 * nd0 + 2) * sizeof(int64_t));
+    int64_t* al = (int64_t*)malloc((size_t)(2 * na0 + 2) * sizeof(int64_t));
+    orc_btree_traverse(bt, dl, al);
+
+    blk_dense* dense = (blk_dense*)malloc((size_t)(nd0 + na0 + 1) * sizeof(blk_dense));
+    blk_lr* lr = (blk_lr*)malloc((size_t)(na0 + 1) * sizeof(blk_lr));
+    int64_t nd = 0, na = 0;
+    /* dense blocks, hmatrix.jl:180-186 */
+    for (int64_t b = 0; b < nd0; b++) {
+        const orc_node* a = &T->nodes[dl[2 * b]];
+        const orc_node* s = &S->nodes[dl[2 * b + 1]];
+        int64_t mm = a->end - a->start, nn = s->end - s->start;
+        double* D = (double*)malloc((size_t)(mm * nn) * sizeof(double));
+        for (int64_t i = 0; i < mm; i++)
+            for (int64_t j = 0; j < nn; j++)
+                D[i * nn + j] = orc_kernel_entry(K, T->index_map[a->start - 1 + i] - 1,
+                                                 S->index_map[s->start - 1 + j] - 1);
+        dense[nd++] = (blk_dense){a->start, a->end - 1, s->start, s->end - 1, D};
+    }
+    /* approximable blocks, hmatrix.jl:189-214 */
+    for (int64_t b = 0; b < na0; b++) {
+        const orc_node* a = &T->nodes[al[2 * b]];
+        const orc_node* s = &S->nodes[al[2 * b + 1]];
+        int64_t mm = a->end - a->start, nn = s->end - s->start;
+        int64_t* rows = (int64_t*)malloc((size_t)mm * sizeof(int64_t));
+        int64_t* cols = (int64_t*)malloc((size_t)nn * sizeof(int64_t));
+        for (int64_t i = 0; i < mm; i++) rows[i] = T->index_map[a->start - 1 + i] - 1;
+        for (int64_t j = 0; j < nn; j++) cols[j] = S->index_map[s->start - 1 + j] - 1;
+        double *Uc, *Vc;
+        int64_t k = orc_aca_plus(K, rows, mm, cols, nn, eps / 10.0, 0, &Uc, &Vc);
+        if (svd_recompress) {
+            double *U2, *V2;
+            int64_t r = orc_svd_recompress(mm, nn, k, Uc, Vc, eps, &U2, &V2);
+            free(Uc);
+            free(Vc);
+            Uc = U2;
+            Vc = V2;
+            k = r;
+        }
+        if (mm * k + k * nn < mm * nn) {
+            double* Ur = (double*)malloc((size_t)(mm * (k ? k : 1)) * sizeof(double));
+            double* Vr = (double*)malloc((size_t)(nn * (k ? k : 1)) * sizeof(double));
+            for (int64_t i = 0; i < mm; i++)
+                for (int64_t q = 0; q < k; q++) Ur[i * k + q] = Uc[i + q * mm];
+            for (int64_t q = 0; q < k; q++)
+                for (int64_t j = 0; j < nn; j++) Vr[q * nn + j] = Vc[q + j * k];
+            lr[na++] = (blk_lr){a->start, a->end - 1, s->start, s->end - 1, k, Ur, Vr};
+        } else { /* storage fallback: appended after all direct blocks, :208-212 */
+            double* D = (double*)malloc((size_t)(mm * nn) * sizeof(double));
+            for (int64_t i = 0; i < mm; i++)
+                for (int64_t j = 0; j < nn; j++) D[i * nn + j] = orc_kernel_entry(K, rows[i], cols[j]);
+            dense[nd++] = (blk_dense){a->start, a->end - 1, s->start, s->end - 1, D};
+        }
+        free(Uc);
+        free(Vc);
+        free(rows);
+        free(cols);
+    }
+    free(dl);
+    free(al);
+    orc_btree_free(bt);
+
+    /* flatten, hmatrix.jl:95-147 */
+    orc_hmat* h = (orc_hmat*)calloc(1, sizeof(orc_hmat));
+    h->m = T->nidx;
+    h->n = S->nidx;
+    h->nd = nd;
+    h->na = na;
+    h->tmap = (int64_t*)malloc((size_t)h->m * sizeof(int64_t));
+    h->smap = (int64_t*)malloc((size_t)h->n * sizeof(int64_t));
+    memcpy(h->tmap, T->index_map, (size_t)h->m * sizeof(int64_t));
+    memcpy(h->smap, S->index_map, (size_t)h->n * sizeof(int64_t));
+    int64_t dlen = 0, ulen = 0, vlen = 0, R = 0;
+    for (int64_t b = 0; b < nd; b++)
+        dlen += (dense[b].r1 - dense[b].r0 + 1) * (dense[b].c1 - dense[b].c0 + 1);
+    for (int64_t b = 0; b < na; b++) {
+        ulen += (lr[b].r1 - lr[b].r0 + 1) * lr[b].k;
+        vlen += (lr[b].c1 - lr[b].c0 + 1) * lr[b].k;
+        R += lr[b].k;
+    }
+    h->R = R;
+    h->dlen = dlen;
+    h->ulen = ulen;
+    h->vlen = vlen;
+    h->D = (double*)malloc((size_t)(dlen ? dlen : 1) * sizeof(double));
+    h->U = (double*)malloc((size_t)(ulen ? ulen : 1) * sizeof(double));
+    h->V = (double*)malloc((size_t)(vlen ? vlen : 1) * sizeof(double));
+    h->dense_idx = (int64_t*)malloc((size_t)(5 * nd + 1) * sizeof(int64_t));
+    h->U_idx = (int64_t*)malloc((size_t)(5 * na + 1) * sizeof(int64_t));
+    h->V_idx = (int64_t*)malloc((size_t)(3 * R + 1) * sizeof(int64_t));
+    h->ranks = (int64_t*)malloc((size_t)(na + 1) * sizeof(int64_t));
+    int64_t off = 0;
+    for (int64_t b = 0; b < nd; b++) {
+        int64_t len = (dense[b].r1 - dense[b].r0 + 1) * (dense[b].c1 - dense[b].c0 + 1);
+        int64_t* e = h->dense_idx + 5 * b;
+        e[0] = dense[b].r0;
+        e[1] = dense[b].r1;
+        e[2] = dense[b].c0;
+        e[3] = dense[b].c1;
+        e[4] = off;
+        memcpy(h->D + off, dense[b].data, (size_t)len * sizeof(double));
+        off += len;
+        free(dense[b].data);
+    }
+    int64_t uoff = 0, voff = 0, vrow = 0, vcol = 0;
+    for (int64_t b = 0; b < na; b++) {
+        int64_t mm = lr[b].r1 - lr[b].r0 + 1, nn = lr[b].c1 - lr[b].c0 + 1, k = lr[b].k;
+        int64_t* e = h->U_idx + 5 * b;
+        e[0] = lr[b].r0;
+        e[1] = lr[b].r1;
+        e[2] = vrow + 1; /* rows 3-4 rewritten to the Vx range, :127-128 */
+        e[3] = vrow + k;
+        e[4] = uoff;
+        memcpy(h->U + uoff, lr[b].U, (size_t)(mm * k) * sizeof(double));
+        memcpy(h->V + voff, lr[b].V, (size_t)(nn * k) * sizeof(double));
+        for (int64_t q = 0; q < k; q++) { /* V_block_indices, :112-125 */
+            int64_t* f = h->V_idx + 3 * vcol;
+            f[0] = lr[b].c0;
+            f[1] = lr[b].c1;
+            f[2] = voff + q * nn;
+            vcol++;
+        }
+        h->ranks[b] = k;
+        uoff += mm * k;
+        voff += nn * k;
+        vrow += k;
+        free(lr[b].U);
+        free(lr[b].V);
+    }
+    free(dense);
+    free(lr);
+    return h;
+}
+
+void orc_hmatrix_sizes(const orc_hmat* h, int64_t* s) {
+    s[0] = h->m;
+    s[1] = h->n;
+    s[2] = h->nd;
+    s[3] = h->na;
+    s[4] = h->R;
+    s[5] = h->dlen;
+    s[6] = h->ulen;
+    s[7] = h->vlen;
+}
+
+void orc_hmatrix_export(const orc_hmat* h, int64_t* tmap, int64_t* smap, int64_t* dense_idx,
+                        int64_t* U_idx, int64_t* V_idx, double* D, double* U, double* V) {
+    if (tmap) memcpy(tmap, h->tmap, (size_t)h->m * sizeof(int64_t));
+    if (smap) memcpy(smap, h->smap, (size_t)h->n * sizeof(int64_t));
+    if (dense_idx) memcpy(dense_idx, h->dense_idx, (size_t)(5 * h->nd) * sizeof(int64_t));
+    if (U_idx) memcpy(U_idx, h->U_idx, (size_t)(5 * h->na) * sizeof(int64_t));
+    if (V_idx) memcpy(V_idx, h->V_idx, (size_t)(3 * h->R) * sizeof(int64_t));
+    if (D) memcpy(D, h->D, (size_t)h->dlen * sizeof(double));
+    if (U) memcpy(U, h->U, (size_t)h->ulen * sizeof(double));
+    if (V) memcpy(V, h->V, (size_t)h->vlen * sizeof(double));
+}
+
+void orc_hmatrix_ranks(const orc_hmat* h, int64_t* ranks) {
+    memcpy(ranks, h->ranks, (size_t)h->na * sizeof(int64_t));
+}
+
+void orc_hmatrix_free(orc_hmat* h) {
+    if (!h) return;
+    free(h->tmap);
+    free(h->smap);
+    free(h->dense_idx);
+    free(h->U_idx);
+    free(h->V_idx);
+    free(h->D);
+    free(h->U);
+    free(h->V);
+    free(h->ranks);
+    free(h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Products                                                                   */
+
+/* mult.jl:6-32 over the flattened layout (dense blocks in list order, then
+ * U*(V*x) per low-rank block in list order, then the target scatter). */
+void orc_mul_layout_partial(const int64_t* smap, const int64_t* dense_idx, const double* D,
+                            int64_t d0, int64_t d1, const int64_t* U_idx, const double* U,
+                            int64_t a0, int64_t a1, const int64_t* V_idx, const double* V,
+                            const double* x, double* xo, int64_t n, double* y) {
+    for (int64_t j = 0; j < n; j++) xo[j] = x[smap[j] - 1]; /* mult.jl:10 */
+    for (int64_t b = d0; b < d1; b++) {                     /* mult.jl:13-18 */
+        const int64_t* e = dense_idx + 5 * b;
+        int64_t nn = e[3] - e[2] + 1;
+        const double* blk = D + e[4];
+        const double* xs = xo + (e[2] - 1);
+        for (int64_t r = e[0]; r <= e[1]; r++) {
+            const double* row = blk + (r - e[0]) * nn;
+            double s = 0.0;
+            for (int64_t j = 0; j < nn; j++) s += row[j] * xs[j];
+            y[r - 1] += s;
+        }
+    }
+    double vx[4096];
+    for (int64_t b = a0; b < a1; b++) { /* mult.jl:21-27 */
+        const int64_t* e = U_idx + 5 * b;
+        int64_t k = e[3] - e[2] + 1;
+        double* t = k <= 4096 ? vx : (double*)malloc((size_t)k * sizeof(double));
+        for (int64_t q = 0; q < k; q++) {
+            const int64_t* f = V_idx + 3 * (e[2] - 1 + q);
+            int64_t nn = f[1] - f[0] + 1;
+            const double* vr = V + f[2];
+            const double* xs = xo + (f[0] - 1);
+            double s = 0.0;
+            for (int64_t j = 0; j < nn; j++) s += vr[j] * xs[j];
+            t[q] = s;
+        }
+        const double* ub = U + e[4];
+        for (int64_t r = e[0]; r <= e[1]; r++) {
+            const double* row = ub + (r - e[0]) * k;
+            double s = 0.0;
+            for (int64_t q = 0; q < k; q++) s += row[q] * t[q];
+            y[r - 1] += s;
+        }
+        if (t != vx) free(t);
+    }
+}
+
+void orc_mul_layout(int64_t m, int64_t n, const int64_t* tmap, const int64_t* smap, int64_t nd,
+                    const int64_t* dense_idx, const double* D, int64_t na, const int64_t* U_idx,
+                    const double* U, int64_t R, const int64_t* V_idx, const double* V,
+                    const double* x, double* y) {
+    (void)R;
+    double* xo = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+    double* yp = (double*)calloc((size_t)(m ? m : 1), sizeof(double));
+    orc_mul_layout_partial(smap, dense_idx, D, 0, nd, U_idx, U, 0, na, V_idx, V, x, xo, n, yp);
+    for (int64_t i = 0; i < m; i++) y[tmap[i] - 1] = yp[i]; /* mult.jl:30 */
+    free(xo);
+    free(yp);
+}
+
+/* Direct FP64 sum; each entry is evaluated once and applied to every column.
+ * Rows are independent, so they are split over OpenMP threads. */
+void orc_dense_matvec(const orc_kernel* K, const int64_t* rows, int64_t nrows, int64_t nrhs,
+                      const double* x, double* y) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < nrows; r++) {
+        double acc[64];
+        for (int64_t c0 = 0; c0 < nrhs; c0 += 64) {
+            int64_t nc = nrhs - c0 < 64 ? nrhs - c0 : 64;
+            for (int64_t c = 0; c < nc; c++) acc[c] = 0.0;
+            for (int64_t j = 0; j < K->n; j++) {
+                double kij = orc_kernel_entry(K, rows[r], j);
+                for (int64_t c = 0; c < nc; c++) acc[c] += kij * x[j + (c0 + c) * K->n];
+            }
+            for (int64_t c = 0; c < nc; c++) y[r + (c0 + c) * nrows] = acc[c];
+        }
+    }
+}

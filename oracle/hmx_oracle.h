@@ -1,0 +1,121 @@
+/*
+ * hmx_oracle.h -- CPU restatement of HMatrixGPU.jl (v0.1.0) used ONLY as a test
+ * checker.  Nothing in the product library links, loads or calls this code; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * All floating point is FP64 with contraction disabled (-ffp-contract=off) so
+ * that the cluster tree / block tree follow Julia 1.10 semantics bit for bit.
+ * Indices exported in the reference's format are 1-based and inclusive (block
+ * tables) or 1-based with an exclusive end (tree nodes), exactly as the Julia
+ * structs store them.
+ */
+#ifndef HMX_ORACLE_H
+#define HMX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- kernels (matrix entries) ------------------------------------------ */
+enum {
+    ORC_K_LAPLACE = 1, /* 1/|x-y|, 0 where the points coincide (diagonal)   */
+    ORC_K_LOG2D = 2,   /* -log(d)/(2 pi), 1.0 where d < 1e-12 (test_hmatrix.jl:17-20) */
+    ORC_K_MATRIX = 3,  /* explicit column-major matrix                        */
+    ORC_K_VEC3 = 4     /* R[j]/r^2, m x 3n, 1.0 where r < 1e-12 (test_hmatrix_vector.jl:14-19) */
+};
+
+typedef struct orc_kernel {
+    int32_t kind;
+    int64_t m, n;          /* matrix size                                  */
+    const double* xt;      /* 3 x (#target points), column-major           */
+    const double* xs;      /* 3 x (#source points), column-major           */
+    const double* mat;     /* ORC_K_MATRIX: column-major, leading dim ld   */
+    int64_t ld;
+} orc_kernel;
+
+/* Entry K[i,j] with 0-based indices. */
+double orc_kernel_entry(const orc_kernel* K, int64_t i, int64_t j);
+
+/* Julia LinearAlgebra.generic_norm2 of a 3-vector (the path tree.jl:86-87 and
+ * block.jl:63 take for length-3 arrays). */
+double orc_norm3(const double v[3]);
+
+/* ---- cluster tree: tree.jl:45-137 -------------------------------------- */
+typedef struct orc_tree orc_tree;
+/* Returns NULL on error; *err: 1 = dims<=0 (tree.jl:46-48), 2 = empty cluster
+ * (the reference throws reducing over an empty point set), 3 = bad args. */
+orc_tree* orc_tree_build(const double* xyz, int64_t npts, int64_t leaf, int64_t dims,
+                         int32_t* err);
+int64_t orc_tree_nnodes(const orc_tree* t);
+int64_t orc_tree_nindex(const orc_tree* t);
+/* Node arrays are in pre-order (node, left subtree, right subtree); child ids
+ * are -1 for leaves.  start/end are the Julia fields (1-based, end exclusive). */
+void orc_tree_export(const orc_tree* t, int64_t* index_map, int64_t* start, int64_t* end,
+                     double* center, double* radius, int64_t* left, int64_t* right,
+                     int8_t* is_leaf);
+/* info(::ClusterTree), tree.jl:153-187 */
+void orc_tree_info(const orc_tree* t, int64_t* depth, int64_t* max_points, int64_t* min_points);
+void orc_tree_free(orc_tree* t);
+
+/* ---- block tree: block.jl:31-155 --------------------------------------- */
+typedef struct orc_btree orc_btree;
+orc_btree* orc_btree_build(const orc_tree* target, const orc_tree* source, double eta);
+void orc_btree_merge(orc_btree* b);                      /* block.jl:98-112  */
+int64_t orc_btree_nnodes(const orc_btree* b);
+void orc_btree_export(const orc_btree* b, int64_t* tnode, int64_t* snode, int8_t* adm,
+                      int8_t* dense, int64_t* left, int64_t* right);
+/* traverse(), block.jl:127-155: counts first, then the (target,source) node ids */
+void orc_btree_traverse_counts(const orc_btree* b, int64_t* ndirect, int64_t* napprox);
+void orc_btree_traverse(const orc_btree* b, int64_t* direct /*2 x nd*/, int64_t* approx /*2 x na*/);
+void orc_btree_free(orc_btree* b);
+
+/* ---- ACA+ and recompression: aca.jl:9-143 ------------------------------ */
+/* rows/cols are 0-based global indices into K.  U is m x k column-major
+ * (column i = left_vectors[i]), V is k x n column-major.  Returns k; the
+ * buffers are malloc'd and released with orc_free. */
+int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const int64_t* cols,
+                     int64_t n, double eps, int64_t max_iter, double** U, double** V);
+/* SVD_recompress(U, V, eps): returns the truncated rank r; U2 m x r, V2 r x n. */
+int64_t orc_svd_recompress(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
+                           double eps, double** U2, double** V2);
+/* Building blocks, exposed for testing against LAPACK (scipy). */
+void orc_qr(int64_t m, int64_t k, double* A, double* tau);
+void orc_qr_apply(int64_t m, int64_t k, const double* A, const double* tau, int64_t ncols,
+                  double* B /* m x ncols, rows >= k may be zero */);
+void orc_jacobi_svd(int64_t k, double* A /* in: M, out: W */, double* sig, double* Z);
+void orc_free(void* p);
+
+/* ---- H-matrix: hmatrix.jl:72-217 + flattened layout -------------------- */
+typedef struct orc_hmat orc_hmat;
+orc_hmat* orc_hmatrix_build(const orc_kernel* K, const orc_tree* target, const orc_tree* source,
+                            double eta, double eps, int32_t svd_recompress);
+/* sizes[0..7] = m, n, n_dense, n_lowrank, R (total rank), dense_len, U_len, V_len */
+void orc_hmatrix_sizes(const orc_hmat* h, int64_t* sizes);
+void orc_hmatrix_export(const orc_hmat* h, int64_t* tmap, int64_t* smap, int64_t* dense_idx,
+                        int64_t* U_idx, int64_t* V_idx, double* D, double* U, double* V);
+/* per low-rank block rank (in approx order) */
+void orc_hmatrix_ranks(const orc_hmat* h, int64_t* ranks);
+void orc_hmatrix_free(orc_hmat* h);
+
+/* CPU product mult.jl:6-32 over the flattened layout: y = H x (one column). */
+void orc_mul_layout(int64_t m, int64_t n, const int64_t* tmap, const int64_t* smap,
+                    int64_t nd, const int64_t* dense_idx, const double* D, int64_t na,
+                    const int64_t* U_idx, const double* U, int64_t R, const int64_t* V_idx,
+                    const double* V, const double* x, double* y);
+/* The same product restricted to dense block range [d0,d1) and low-rank range
+ * [a0,a1) (used to time a bounded sample); y accumulates in permuted order
+ * and is NOT scattered. */
+void orc_mul_layout_partial(const int64_t* smap, const int64_t* dense_idx, const double* D,
+                            int64_t d0, int64_t d1, const int64_t* U_idx, const double* U,
+                            int64_t a0, int64_t a1, const int64_t* V_idx, const double* V,
+                            const double* x, double* xo_scratch, int64_t n, double* yperm);
+/* Direct sum y_i = sum_j K[i,j] x_j for the listed rows (0-based), nrhs columns
+ * (x column-major n x nrhs, y column-major nrows x nrhs). */
+void orc_dense_matvec(const orc_kernel* K, const int64_t* rows, int64_t nrows, int64_t nrhs,
+                      const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

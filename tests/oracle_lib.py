@@ -1,0 +1,326 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the thing measured
+or shipped.  Every helper returns numpy arrays in the reference's own format
+(1-based indices, inclusive block ranges, row-major flattened payloads).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+K_LAPLACE, K_LOG2D, K_MATRIX, K_VEC3 = 1, 2, 3, 4
+
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_i8p = C.POINTER(C.c_int8)
+
+
+class Kernel(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("m", C.c_int64), ("n", C.c_int64),
+                ("xt", _f64p), ("xs", _f64p), ("mat", _f64p), ("ld", C.c_int64)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "orc_norm3": (C.c_double, [_f64p]),
+        "orc_kernel_entry": (C.c_double, [C.POINTER(Kernel), C.c_int64, C.c_int64]),
+        "orc_tree_build": (vp, [_f64p, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int32)]),
+        "orc_tree_nnodes": (C.c_int64, [vp]),
+        "orc_tree_nindex": (C.c_int64, [vp]),
+        "orc_tree_export": (None, [vp, _i64p, _i64p, _i64p, _f64p, _f64p, _i64p, _i64p, _i8p]),
+        "orc_tree_info": (None, [vp, _i64p, _i64p, _i64p]),
+        "orc_tree_free": (None, [vp]),
+        "orc_btree_build": (vp, [vp, vp, C.c_double]),
+        "orc_btree_merge": (None, [vp]),
+        "orc_btree_nnodes": (C.c_int64, [vp]),
+        "orc_btree_export": (None, [vp, _i64p, _i64p, _i8p, _i8p, _i64p, _i64p]),
+        "orc_btree_traverse_counts": (None, [vp, _i64p, _i64p]),
+        "orc_btree_traverse": (None, [vp, _i64p, _i64p]),
+        "orc_btree_free": (None, [vp]),
+        "orc_aca_plus": (C.c_int64, [C.POINTER(Kernel), _i64p, C.c_int64, _i64p, C.c_int64,
+                                     C.c_double, C.c_int64, C.POINTER(_f64p), C.POINTER(_f64p)]),
+        "orc_svd_recompress": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, _f64p, _f64p,
+                                           C.c_double, C.POINTER(_f64p), C.POINTER(_f64p)]),
+        "orc_qr": (None, [C.c_int64, C.c_int64, _f64p, _f64p]),
+        "orc_qr_apply": (None, [C.c_int64, C.c_int64, _f64p, _f64p, C.c_int64, _f64p]),
+        "orc_jacobi_svd": (None, [C.c_int64, _f64p, _f64p, _f64p]),
+        "orc_free": (None, [vp]),
+        "orc_hmatrix_build": (vp, [C.POINTER(Kernel), vp, vp, C.c_double, C.c_double, C.c_int32]),
+        "orc_hmatrix_sizes": (None, [vp, _i64p]),
+        "orc_hmatrix_export": (None, [vp, _i64p, _i64p, _i64p, _i64p, _i64p, _f64p, _f64p, _f64p]),
+        "orc_hmatrix_ranks": (None, [vp, _i64p]),
+        "orc_hmatrix_free": (None, [vp]),
+        "orc_mul_layout": (None, [C.c_int64, C.c_int64, _i64p, _i64p, C.c_int64, _i64p, _f64p,
+                                  C.c_int64, _i64p, _f64p, C.c_int64, _i64p, _f64p, _f64p, _f64p]),
+        "orc_mul_layout_partial": (None, [_i64p, _i64p, _f64p, C.c_int64, C.c_int64, _i64p, _f64p,
+                                          C.c_int64, C.c_int64, _i64p, _f64p, _f64p, _f64p,
+                                          C.c_int64, _f64p]),
+        "orc_dense_matvec": (None, [C.POINTER(Kernel), _i64p, C.c_int64, C.c_int64, _f64p, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def p64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def pi64(a):
+    return a.ctypes.data_as(_i64p)
+
+
+def pi8(a):
+    return a.ctypes.data_as(_i8p)
+
+
+def make_kernel(kind, xt=None, xs=None, mat=None, m=None, n=None):
+    """Kernel descriptor; the arrays are kept alive on the returned object."""
+    k = Kernel()
+    k.kind = kind
+    keep = []
+    if xt is not None:
+        xt = np.ascontiguousarray(np.asarray(xt, dtype=np.float64).reshape(-1, 3))
+        keep.append(xt)
+        k.xt = p64(xt)
+        m = m if m is not None else xt.shape[0]
+    if xs is not None:
+        xs = np.ascontiguousarray(np.asarray(xs, dtype=np.float64).reshape(-1, 3))
+        keep.append(xs)
+        k.xs = p64(xs)
+        n = n if n is not None else xs.shape[0]
+    if mat is not None:
+        mat = np.asfortranarray(mat, dtype=np.float64)
+        keep.append(mat)
+        k.mat = p64(mat)
+        k.ld = mat.shape[0]
+        m, n = mat.shape
+    k.m, k.n = m, n
+    k._keep = keep
+    return k
+
+
+def points_cm(pts):
+    """(N,3) points -> contiguous buffer equal to the Julia 3xN column-major matrix."""
+    return np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+
+
+class Tree:
+    def __init__(self, pts, leaf, dims=1):
+        pts = points_cm(pts)
+        self.pts = pts
+        err = C.c_int32(0)
+        self.h = lib.orc_tree_build(p64(pts), pts.shape[0], leaf, dims, C.byref(err))
+        if not self.h:
+            raise ValueError("orc_tree_build failed with code %d" % err.value)
+        nn = lib.orc_tree_nnodes(self.h)
+        ni = lib.orc_tree_nindex(self.h)
+        self.index_map = np.zeros(ni, np.int64)
+        self.start = np.zeros(nn, np.int64)
+        self.end = np.zeros(nn, np.int64)
+        self.center = np.zeros((nn, 3))
+        self.radius = np.zeros(nn)
+        self.left = np.zeros(nn, np.int64)
+        self.right = np.zeros(nn, np.int64)
+        self.is_leaf = np.zeros(nn, np.int8)
+        lib.orc_tree_export(self.h, pi64(self.index_map), pi64(self.start), pi64(self.end),
+                            p64(self.center), p64(self.radius), pi64(self.left),
+                            pi64(self.right), pi8(self.is_leaf))
+
+    def info(self):
+        d, mx, mn = C.c_int64(), C.c_int64(), C.c_int64()
+        lib.orc_tree_info(self.h, C.byref(d), C.byref(mx), C.byref(mn))
+        return {"depth": d.value, "max_points": mx.value, "min_points": mn.value}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_tree_free(self.h)
+            self.h = None
+
+
+class BlockTree:
+    def __init__(self, T, S, eta, merge=False):
+        self.T, self.S = T, S
+        self.h = lib.orc_btree_build(T.h, S.h, eta)
+        if merge:
+            lib.orc_btree_merge(self.h)
+        self._export()
+
+    def merge(self):
+        lib.orc_btree_merge(self.h)
+        self._export()
+
+    def _export(self):
+        nn = lib.orc_btree_nnodes(self.h)
+        self.tnode = np.zeros(nn, np.int64)
+        self.snode = np.zeros(nn, np.int64)
+        self.adm = np.zeros(nn, np.int8)
+        self.dense = np.zeros(nn, np.int8)
+        self.left = np.zeros(nn, np.int64)
+        self.right = np.zeros(nn, np.int64)
+        lib.orc_btree_export(self.h, pi64(self.tnode), pi64(self.snode), pi8(self.adm),
+                             pi8(self.dense), pi64(self.left), pi64(self.right))
+
+    def traverse(self):
+        nd, na = C.c_int64(), C.c_int64()
+        lib.orc_btree_traverse_counts(self.h, C.byref(nd), C.byref(na))
+        d = np.zeros((max(nd.value, 1), 2), np.int64)
+        a = np.zeros((max(na.value, 1), 2), np.int64)
+        lib.orc_btree_traverse(self.h, pi64(d), pi64(a))
+        return d[:nd.value], a[:na.value]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_btree_free(self.h)
+            self.h = None
+
+
+def _take(ptr, count):
+    if count == 0:
+        lib.orc_free(C.cast(ptr, C.c_void_p))
+        return np.zeros(0)
+    out = np.ctypeslib.as_array(ptr, shape=(count,)).copy()
+    lib.orc_free(C.cast(ptr, C.c_void_p))
+    return out
+
+
+def aca_plus(kernel, rows, cols, eps, max_iter=0):
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    Up, Vp = _f64p(), _f64p()
+    k = lib.orc_aca_plus(C.byref(kernel), pi64(rows), len(rows), pi64(cols), len(cols), eps,
+                         max_iter, C.byref(Up), C.byref(Vp))
+    U = _take(Up, len(rows) * k).reshape(k, len(rows)).T
+    V = _take(Vp, len(cols) * k).reshape(len(cols), k).T
+    return U, V
+
+
+def svd_recompress(U, V, eps):
+    U = np.asfortranarray(U, np.float64)
+    V = np.asfortranarray(V, np.float64)
+    m, k = U.shape
+    n = V.shape[1]
+    Up, Vp = _f64p(), _f64p()
+    r = lib.orc_svd_recompress(m, n, k, p64(U), p64(V), eps, C.byref(Up), C.byref(Vp))
+    U2 = _take(Up, m * r).reshape(r, m).T
+    V2 = _take(Vp, n * r).reshape(n, r).T
+    return U2, V2
+
+
+def qr(A):
+    A = np.asfortranarray(A, np.float64).copy(order="F")
+    m, k = A.shape
+    tau = np.zeros(k)
+    lib.orc_qr(m, k, A.ctypes.data_as(_f64p), p64(tau))
+    return A, tau
+
+
+def jacobi_svd(M):
+    A = np.asfortranarray(M, np.float64).copy(order="F")
+    k = A.shape[0]
+    sig = np.zeros(k)
+    Z = np.zeros((k, k), order="F")
+    lib.orc_jacobi_svd(k, A.ctypes.data_as(_f64p), p64(sig), Z.ctypes.data_as(_f64p))
+    return A, sig, Z
+
+
+class HMatrix:
+    """Oracle H-matrix in the reference's flattened layout (hmatrix.jl:95-147)."""
+
+    def __init__(self, kernel, T, S, eta, eps, svd_recompress=True):
+        self.kernel = kernel
+        self.h = lib.orc_hmatrix_build(C.byref(kernel), T.h, S.h, eta, eps, int(svd_recompress))
+        s = np.zeros(8, np.int64)
+        lib.orc_hmatrix_sizes(self.h, pi64(s))
+        self.m, self.n, self.nd, self.na, self.R, dl, ul, vl = [int(v) for v in s]
+        self.target_index_map = np.zeros(self.m, np.int64)
+        self.source_index_map = np.zeros(self.n, np.int64)
+        self.dense_block_indices = np.zeros((max(self.nd, 1), 5), np.int64)
+        self.U_block_indices = np.zeros((max(self.na, 1), 5), np.int64)
+        self.V_block_indices = np.zeros((max(self.R, 1), 3), np.int64)
+        self.dense_blocks = np.zeros(max(dl, 1))
+        self.U_matrices = np.zeros(max(ul, 1))
+        self.V_matrices = np.zeros(max(vl, 1))
+        lib.orc_hmatrix_export(self.h, pi64(self.target_index_map), pi64(self.source_index_map),
+                               pi64(self.dense_block_indices), pi64(self.U_block_indices),
+                               pi64(self.V_block_indices), p64(self.dense_blocks),
+                               p64(self.U_matrices), p64(self.V_matrices))
+        self.dense_block_indices = self.dense_block_indices[:self.nd]
+        self.U_block_indices = self.U_block_indices[:self.na]
+        self.V_block_indices = self.V_block_indices[:self.R]
+        self.dense_blocks = self.dense_blocks[:dl]
+        self.U_matrices = self.U_matrices[:ul]
+        self.V_matrices = self.V_matrices[:vl]
+        self.ranks = np.zeros(max(self.na, 1), np.int64)
+        lib.orc_hmatrix_ranks(self.h, pi64(self.ranks))
+        self.ranks = self.ranks[:self.na]
+
+    def matvec(self, x):
+        return mul_layout(self, x)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_hmatrix_free(self.h)
+            self.h = None
+
+
+def mul_layout(L, x):
+    """mult.jl:6-32 over any object with the reference's flattened fields."""
+    x = np.ascontiguousarray(x, np.float64)
+    m = len(L.target_index_map)
+    n = len(L.source_index_map)
+    y = np.zeros(m)
+    di = np.ascontiguousarray(L.dense_block_indices, np.int64).reshape(-1, 5)
+    ui = np.ascontiguousarray(L.U_block_indices, np.int64).reshape(-1, 5)
+    vi = np.ascontiguousarray(L.V_block_indices, np.int64).reshape(-1, 3)
+    D = np.ascontiguousarray(L.dense_blocks, np.float64)
+    U = np.ascontiguousarray(L.U_matrices, np.float64)
+    V = np.ascontiguousarray(L.V_matrices, np.float64)
+    if D.size == 0:
+        D = np.zeros(1)
+    if U.size == 0:
+        U = np.zeros(1)
+    if V.size == 0:
+        V = np.zeros(1)
+    tm = np.ascontiguousarray(L.target_index_map, np.int64)
+    sm = np.ascontiguousarray(L.source_index_map, np.int64)
+    lib.orc_mul_layout(m, n, pi64(tm), pi64(sm), di.shape[0], pi64(di), p64(D), ui.shape[0],
+                       pi64(ui), p64(U), vi.shape[0], pi64(vi), p64(V), p64(x), p64(y))
+    return y
+
+
+def dense_matvec(kernel, x, rows=None):
+    x = np.asfortranarray(x, np.float64)
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    if rows is None:
+        rows = np.arange(kernel.m, dtype=np.int64)
+    rows = np.ascontiguousarray(rows, np.int64)
+    y = np.zeros((len(rows), nrhs), order="F")
+    lib.orc_dense_matvec(C.byref(kernel), pi64(rows), len(rows), nrhs, x.ctypes.data_as(_f64p),
+                         y.ctypes.data_as(_f64p))
+    return y[:, 0] if x.ndim == 1 else y
+
+
+def kernel_matrix(kernel, rows=None, cols=None):
+    rows = np.arange(kernel.m) if rows is None else rows
+    cols = np.arange(kernel.n) if cols is None else cols
+    out = np.empty((len(rows), len(cols)))
+    for a, i in enumerate(rows):
+        for b, j in enumerate(cols):
+            out[a, b] = lib.orc_kernel_entry(C.byref(kernel), int(i), int(j))
+    return out

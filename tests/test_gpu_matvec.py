@@ -1,0 +1,131 @@
+"""GPU parity of the H-matrix product (hmx_matvec through the C ABI).
+
+Reference behaviour: mul!(result, H::HMatrix, x), mult.jl:159-189, whose value
+is the CPU product of mult.jl:6-32 over the same blocks.  The checker is the
+oracle's restatement of that CPU product (oracle_lib.mul_layout) evaluated on
+the layout the product itself exports, plus the FP64 direct sum.
+
+Tolerances (relative 2-norm error):
+  FP64 storage, any RHS count ........ 1e-12 vs the oracle H-matvec
+  FP32 storage (FP32 accumulate) ..... 1e-5  vs the oracle H-matvec
+  BF16 storage (FP32 accumulate) ..... 2e-2  vs the oracle H-matvec (reported, no claim)
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import layout_ns, rel, uniform_points
+
+pytestmark = pytest.mark.gpu
+
+N1, LEAF1 = 4096, 32
+
+
+@pytest.fixture(scope="module")
+def c1(cuda):
+    import hmatrixgpu as H
+    print("[c1] building N=%d leaf=%d" % (N1, LEAF1), flush=True)
+    pts = uniform_points(N1, 1)
+    X = H.ClusterTree(pts, max_points_per_leaf=LEAF1)
+    hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=1e-6, storage=("f64", "f32", "bf16"))
+    return pts, hm, layout_ns(hm)
+
+
+def _col_major(torch, arr, dev, dtype):
+    t = torch.tensor(np.asarray(arr), dtype=dtype, device=dev)
+    return t.t().contiguous().t() if t.dim() == 2 else t
+
+
+def test_fp64_single_rhs_matches_oracle(c1, cuda):
+    import torch
+    import hmatrixgpu as H
+    pts, hm, lay = c1
+    x = np.random.default_rng(11).standard_normal(N1)
+    y_ref = O.mul_layout(lay, x)
+    y = H.mul_(torch.empty(N1, dtype=torch.float64, device=cuda), hm,
+               torch.tensor(x, device=cuda), storage="f64")
+    torch.cuda.synchronize()
+    assert rel(y.cpu().numpy(), y_ref) <= 1e-12
+    # and the H-matrix approximates the dense kernel product (test_hmatrix.jl:46-50)
+    K = O.make_kernel(O.K_LAPLACE, xt=pts.T, xs=pts.T)
+    assert rel(y.cpu().numpy(), O.dense_matvec(K, x)) <= 1e-6
+
+
+@pytest.mark.parametrize("nrhs", [16, 3, 20])
+def test_fp64_multi_rhs_matches_oracle(c1, cuda, nrhs):
+    import torch
+    import hmatrixgpu as H
+    _, hm, lay = c1
+    X = np.random.default_rng(12 + nrhs).standard_normal((N1, nrhs))
+    Yref = np.stack([O.mul_layout(lay, X[:, c]) for c in range(nrhs)], axis=1)
+    Xt = _col_major(torch, X, cuda, torch.float64)
+    Yt = _col_major(torch, np.zeros((N1, nrhs)), cuda, torch.float64)
+    H.mul_(Yt, hm, Xt, storage="f64")
+    torch.cuda.synchronize()
+    assert rel(Yt.cpu().numpy(), Yref) <= 1e-12
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("bf16", 2e-2)])
+@pytest.mark.parametrize("nrhs", [1, 16])
+def test_low_precision_storage(c1, cuda, storage, tol, nrhs):
+    import torch
+    import hmatrixgpu as H
+    _, hm, lay = c1
+    X = np.random.default_rng(21).standard_normal((N1, nrhs))
+    Yref = np.stack([O.mul_layout(lay, X[:, c]) for c in range(nrhs)], axis=1)
+    Xt = _col_major(torch, X, cuda, torch.float32)
+    Yt = _col_major(torch, np.zeros((N1, nrhs)), cuda, torch.float32)
+    H.mul_(Yt, hm, Xt, storage=storage)
+    torch.cuda.synchronize()
+    err = rel(Yt.cpu().numpy(), Yref)
+    print("storage=%s nrhs=%d rel err %.3e" % (storage, nrhs, err))
+    assert err <= tol
+
+
+def test_leading_dimension_and_determinism(c1, cuda):
+    import torch
+    import hmatrixgpu as H
+    _, hm, lay = c1
+    nrhs, ld = 16, N1 + 37
+    X = np.random.default_rng(5).standard_normal((N1, nrhs))
+    buf = torch.zeros((nrhs, ld), dtype=torch.float64, device=cuda)
+    buf[:, :N1] = torch.tensor(X.T, device=cuda)
+    Xt = buf.t()[:N1]  # stride (1, ld)
+    out1 = torch.full((nrhs, ld), 7.0, dtype=torch.float64, device=cuda)
+    out2 = torch.full((nrhs, ld), 7.0, dtype=torch.float64, device=cuda)
+    H.mul_(out1.t()[:N1], hm, Xt, storage="f64")
+    H.mul_(out2.t()[:N1], hm, Xt, storage="f64")
+    torch.cuda.synchronize()
+    a, b = out1.cpu().numpy(), out2.cpu().numpy()
+    assert np.array_equal(a, b)  # atomic-free: bitwise reproducible
+    assert np.all(a[:, N1:] == 7.0)  # padding beyond m untouched
+    Yref = np.stack([O.mul_layout(lay, X[:, c]) for c in range(nrhs)], axis=1)
+    assert rel(a[:, :N1].T, Yref) <= 1e-12
+
+
+def test_callback_kernel_unit_circle(cuda):
+    """test_hmatrix.jl:10-50: unit circle N=1000, -log(d)/2pi kernel, leaf 64,
+    eta 1.5, eps 1e-6, entries supplied by a host callback."""
+    import torch
+    import hmatrixgpu as H
+    n = 1000
+    ang = np.arange(1, n + 1) * 2 * np.pi / n
+    pts = np.stack([np.sin(ang), np.cos(ang), np.zeros(n)])
+    kern = H.Log2DKernel(pts)
+
+    class Callback:  # any matrix-like object: goes through the host callback
+        shape = kern.shape
+
+        def __getitem__(self, idx):
+            return kern[idx[0].ravel(), idx[1].ravel()]
+
+    X = H.ClusterTree(pts, max_points_per_leaf=64)
+    hm = H.HMatrix(Callback(), X, X, eta=1.5, eps=1e-6, storage=("f64",))
+    assert hm.info()["compression_ratio"] > 3  # test_hmatrix.jl:30-32
+    x = np.random.default_rng(3).random(n)
+    y = H.mul_(torch.empty(n, dtype=torch.float64, device=cuda), hm,
+               torch.tensor(x, device=cuda))
+    torch.cuda.synchronize()
+    Kd = kern[np.arange(n), np.arange(n)]
+    assert np.linalg.norm(y.cpu().numpy() - Kd @ x) <= 1e-5  # test_hmatrix.jl:49-50
+    assert rel(y.cpu().numpy(), O.mul_layout(layout_ns(hm), x)) <= 1e-12
