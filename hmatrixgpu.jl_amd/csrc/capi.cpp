@@ -280,8 +280,30 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
         h->src.xs = h->xs_copy.empty() ? nullptr : h->xs_copy.data();
         h->device = device;
         for (int s = 0; s < 3; s++)
-            if (storage_mask & (1 << s)) hmx::device_build(*h, s, T);
+            if (storage_mask & (1 << s)) hmx::device_build(*h, s);
         *out = new hmx_hmatrix{std::move(h)};
+    });
+}
+
+int hmx_hmatrix_materialize(hmx_hmatrix* hm, int32_t storage_mask, int32_t device) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_hmatrix_materialize: NULL");
+        need(storage_mask >= 0 && storage_mask < 8, "hmx_hmatrix_materialize: bad storage_mask");
+        hmx::HMatrix& h = *hm->h;
+        for (int s = 0; s < 3; s++)
+            if ((storage_mask & (1 << s)) && h.dev[s] && device != h.device)
+                throw hmx::Error(HMX_ERR_STATE, "storage already materialised on another device");
+        h.device = device;
+        for (int s = 0; s < 3; s++)
+            if (storage_mask & (1 << s)) hmx::device_build(h, s);
+    });
+}
+
+int hmx_hmatrix_release(hmx_hmatrix* hm, int32_t storage_mask) {
+    return guarded([&] {
+        need(hm && hm->h, "hmx_hmatrix_release: NULL");
+        for (int s = 0; s < 3; s++)
+            if (storage_mask & (1 << s)) hm->h->dev[s].reset();
     });
 }
 

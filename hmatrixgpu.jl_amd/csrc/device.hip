@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -145,136 +146,242 @@ struct SType<bf16s, 16> {
 constexpr int kWaves = 4;  // waves per workgroup in the panel kernels
 
 // ---- the panel kernel ----------------------------------------------------------
+// Buffer resources (raw, stride 0): loads past num_records return 0, which masks
+// the absent rows of a partial row tile, the padding of the last k-tile and the
+// prefetch past the end of a panel without branches.
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer descriptor
+constexpr uint32_t kOOB = 0xffffffffu;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
+                                             kRsrcWord3);
+}
+
+template <int N>
+struct IdxV {
+    int32_t v[N];
+};
+
+// Column indices of the lane's EPL columns of k-tile kt.
+template <int EPL, int TK>
+__device__ inline IdxV<EPL> load_idx(__amdgpu_buffer_rsrc_t C, int kt, int lg) {
+    IdxV<EPL> r;
+    const uint32_t off = static_cast<uint32_t>((kt * TK + lg * EPL) * 4);
+    if constexpr (EPL == 2) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(C, off, 0, 0);
+        r.v[0] = static_cast<int32_t>(v[0]);
+        r.v[1] = static_cast<int32_t>(v[1]);
+    } else if constexpr (EPL == 4) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(C, off, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) r.v[e] = static_cast<int32_t>(v[e]);
+    } else {
+        auto v0 = __builtin_amdgcn_raw_buffer_load_b128(C, off, 0, 0);
+        auto v1 = __builtin_amdgcn_raw_buffer_load_b128(C, off + 16, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            r.v[e] = static_cast<int32_t>(v0[e]);
+            r.v[4 + e] = static_cast<int32_t>(v1[e]);
+        }
+    }
+    return r;
+}
+
+template <typename T>
+__device__ inline T buf_load(__amdgpu_buffer_rsrc_t R, uint32_t off);
+template <>
+__device__ inline double buf_load<double>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(R, off, 0, 0);
+    return __builtin_bit_cast(double, v);
+}
+template <>
+__device__ inline float buf_load<float>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(R, off, 0, 0));
+}
+template <>
+__device__ inline bf16s buf_load<bf16s>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    return bf16s{__builtin_amdgcn_raw_buffer_load_b16(R, off, 0, 0)};
+}
+
+// One wave's share of a panel: k-tiles wave, wave + kWaves, ... accumulated
+// into red[row][col] (this wave's slice).  Loads are issued one k-tile ahead
+// (A and B) and two k-tiles ahead (column indices).
+template <typename TS, int W, bool LEAF, int NRT>
+__device__ __forceinline__ void panel_body(const Item& it, const TS* panel, const int32_t* colidx,
+                                           const typename SType<TS, W>::type* S, uint32_t s_bytes,
+                                           int wave, int lane,
+                                           typename Store<TS>::acc (*red)[W]) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
+    constexpr uint32_t SB = sizeof(s_t), ROWB = W * sizeof(s_t);
+    const int li = lane & 15, lg = lane >> 4;
+    const int rows = it.rows;
+    const uint32_t slab = static_cast<uint32_t>(rows) * 64u;
+    const __amdgpu_buffer_rsrc_t Ar = make_rsrc(panel + it.data_off, slab * it.ktiles);
+    const __amdgpu_buffer_rsrc_t Sr = make_rsrc(S, s_bytes);
+    const __amdgpu_buffer_rsrc_t Cr =
+        make_rsrc(LEAF ? colidx + it.col_off : colidx, LEAF ? it.ktiles * TK * 4 : 0);
+    uint32_t aoff[NRT];
+    bool aval[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) {
+        const int rr = min(16, rows - 16 * rt);
+        aval[rt] = li < rr;
+        aoff[rt] = static_cast<uint32_t>((16 * rt * TK + (rr * lg + li) * EPL) * sizeof(TS));
+    }
+    const uint32_t bcol = (W == 16 ? li : 0) * SB;
+
+    struct Stage {
+        u32x4 a[NRT];
+        s_t b[EPL];
+    };
+    auto load_stage = [&](int kt, const IdxV<EPL>& ix) {
+        Stage st;
+        const uint32_t base = static_cast<uint32_t>(kt) * slab;
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+            const uint32_t o = aval[rt] ? base + aoff[rt] : kOOB;
+            st.a[rt] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(Ar, o, 0, 0));
+        }
+#pragma unroll
+        for (int e = 0; e < EPL; e++) {
+            uint32_t o;
+            if constexpr (LEAF) {
+                o = static_cast<uint32_t>(ix.v[e]) * ROWB + bcol;
+            } else {
+                const int k = kt * TK + lg * EPL + e;
+                o = k < it.ncols ? static_cast<uint32_t>(it.col_off + k) * ROWB + bcol : kOOB;
+            }
+            st.b[e] = buf_load<s_t>(Sr, o);
+        }
+        return st;
+    };
+
+    using vacc = typename std::conditional<W == 16,
+        typename std::conditional<std::is_same<acc_t, double>::value, f64x4, f32x4>::type,
+        acc_t>::type;
+    vacc acc[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) acc[rt] = vacc{};
+
+    auto compute = [&](const Stage& st) {
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+            const u32x4 raw = st.a[rt];
+            if constexpr (W == 16) {
+                if constexpr (std::is_same<TS, float>::value) {
+                    f32x4 a = __builtin_bit_cast(f32x4, raw);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, st.b[0], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, st.b[1], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, st.b[2], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, st.b[3], acc[rt], 0, 0, 0);
+                } else if constexpr (std::is_same<TS, double>::value) {
+                    typedef double d2 __attribute__((ext_vector_type(2)));
+                    d2 a = __builtin_bit_cast(d2, raw);
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, st.b[0], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, st.b[1], acc[rt], 0, 0, 0);
+                } else {
+                    bf16x8 a = __builtin_bit_cast(bf16x8, raw);
+                    uint16_t bb[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) bb[e] = st.b[e].u;
+                    bf16x8 bv = __builtin_bit_cast(bf16x8, bb);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[rt], 0, 0, 0);
+                }
+            } else {
+                if constexpr (std::is_same<TS, float>::value) {
+                    f32x4 a = __builtin_bit_cast(f32x4, raw);
+                    acc[rt] += a.x * st.b[0] + a.y * st.b[1] + a.z * st.b[2] + a.w * st.b[3];
+                } else if constexpr (std::is_same<TS, double>::value) {
+                    typedef double d2 __attribute__((ext_vector_type(2)));
+                    d2 a = __builtin_bit_cast(d2, raw);
+                    acc[rt] += a.x * st.b[0] + a.y * st.b[1];
+                } else {
+                    uint16_t h[8];
+                    std::memcpy(h, &raw, 16);
+                    acc_t s = 0;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) s += bf2f(h[e]) * st.b[e];
+                    acc[rt] += s;
+                }
+            }
+        }
+    };
+
+    // Software pipeline, unrolled by two so the stage buffers alternate without
+    // register copies: while stage A is consumed, stage B (next k-tile) and the
+    // column indices of the k-tile after it are in flight.  Loads past the end
+    // of the panel (kt >= ktiles) fall outside the buffer and return zeros;
+    // they are never consumed.
+    int kt = wave;
+    IdxV<EPL> ixA{}, ixB{};
+    if constexpr (LEAF) {
+        ixA = load_idx<EPL, TK>(Cr, kt, lg);
+        ixB = load_idx<EPL, TK>(Cr, kt + kWaves, lg);
+    }
+    Stage sA = load_stage(kt, ixA), sB;
+    while (kt < it.ktiles) {
+        if constexpr (LEAF) ixA = load_idx<EPL, TK>(Cr, kt + 2 * kWaves, lg);
+        sB = load_stage(kt + kWaves, ixB);
+        compute(sA);
+        kt += kWaves;
+        if (kt >= it.ktiles) break;
+        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 2 * kWaves, lg);
+        sA = load_stage(kt + kWaves, ixA);
+        compute(sB);
+        kt += kWaves;
+    }
+
+    if constexpr (W == 16) {
+        // MFMA 16x16 output, column li: f32 accumulators hold rows 4*lg+v of the
+        // row tile, the f64 instruction holds rows 4*v+lg.
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = std::is_same<TS, double>::value ? 4 * v + lg : 4 * lg + v;
+                red[16 * rt + row][li] = acc[rt][v];
+            }
+    } else {
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+            acc_t v = acc[rt];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (lg == 0) red[16 * rt + li][0] = v;
+        }
+    }
+}
+
 // LEAF: leaf panel (indexed S rows, output scattered to Y through target_map);
 // otherwise far-field panel (contiguous S rows, output to partial rows or T).
 template <typename TS, int W, bool LEAF, typename TIO>
 __global__ __launch_bounds__(kWaves * 64) void k_panel(
     const Item* __restrict__ items, const TS* __restrict__ panel,
     const int32_t* __restrict__ colidx, typename SType<TS, W>::type* __restrict__ S, int64_t nsrc,
-    typename Store<TS>::acc* __restrict__ partial, const int32_t* __restrict__ tmap0,
-    TIO* __restrict__ Y, int64_t ldy, int32_t rhs0, int32_t nrhs) {
+    uint32_t s_bytes, typename Store<TS>::acc* __restrict__ partial,
+    const int32_t* __restrict__ tmap0, TIO* __restrict__ Y, int64_t ldy, int32_t rhs0,
+    int32_t nrhs) {
     using acc_t = typename Store<TS>::acc;
     using s_t = typename SType<TS, W>::type;
-    constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
     const Item it = items[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lg = lane >> 4;
     const int rows = it.rows;
-    const int nrt = (rows + 15) >> 4;
-
     __shared__ acc_t red[kWaves][64][W];
-
-    if constexpr (W == 16) {
-        using vacc = typename std::conditional<std::is_same<acc_t, double>::value, f64x4, f32x4>::type;
-        vacc acc[4];
-        for (int r = 0; r < 4; r++) acc[r] = vacc{0, 0, 0, 0};
-        for (int kt = wave; kt < it.ktiles; kt += kWaves) {
-            const TS* slab = panel + it.data_off + static_cast<int64_t>(kt) * rows * TK;
-            const int kbase = kt * TK + lg * EPL;
-            // B operand: S[row(k)][li] for the lane's EPL columns
-            s_t b[EPL];
-#pragma unroll
-            for (int e = 0; e < EPL; e++) {
-                int64_t srow;
-                bool ok = true;
-                if constexpr (LEAF) {
-                    srow = colidx[it.col_off + kbase + e];
-                } else {
-                    srow = it.col_off + kbase + e;
-                    ok = (kbase + e) < it.ncols;
-                }
-                if constexpr (std::is_same<s_t, bf16s>::value)
-                    b[e] = ok ? S[srow * 16 + li] : bf16s{0};
-                else
-                    b[e] = ok ? S[srow * 16 + li] : s_t(0);
-            }
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                if (rt < nrt) {
-                    const int rr = min(16, rows - 16 * rt);
-                    u32x4 raw = {0u, 0u, 0u, 0u};
-                    if (li < rr)
-                        raw = *reinterpret_cast<const u32x4*>(slab + 16 * rt * TK +
-                                                               (rr * lg + li) * EPL);
-                    if constexpr (std::is_same<TS, float>::value) {
-                        f32x4 a = __builtin_bit_cast(f32x4, raw);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[0], acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b[1], acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b[2], acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b[3], acc[rt], 0, 0, 0);
-                    } else if constexpr (std::is_same<TS, double>::value) {
-                        typedef double d2 __attribute__((ext_vector_type(2)));
-                        d2 a = __builtin_bit_cast(d2, raw);
-                        acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b[0], acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b[1], acc[rt], 0, 0, 0);
-                    } else {
-                        bf16x8 a = __builtin_bit_cast(bf16x8, raw);
-                        uint16_t bb[8];
-#pragma unroll
-                        for (int e = 0; e < 8; e++) bb[e] = b[e].u;
-                        bf16x8 bv = __builtin_bit_cast(bf16x8, bb);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[rt], 0, 0, 0);
-                    }
-                }
-            }
-        }
-        // MFMA 16x16 output, column li: f32 accumulators hold rows 4*lg+v of the
-        // row tile, the f64 instruction holds rows 4*v+lg.
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++)
-            if (rt < nrt)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int row = std::is_same<TS, double>::value ? 4 * v + lg : 4 * lg + v;
-                    red[wave][16 * rt + row][li] = acc[rt][v];
-                }
-    } else {
-        acc_t acc[4] = {0, 0, 0, 0};
-        for (int kt = wave; kt < it.ktiles; kt += kWaves) {
-            const TS* slab = panel + it.data_off + static_cast<int64_t>(kt) * rows * TK;
-            const int kbase = kt * TK + lg * EPL;
-            acc_t b[EPL];
-#pragma unroll
-            for (int e = 0; e < EPL; e++) {
-                if constexpr (LEAF) {
-                    b[e] = S[colidx[it.col_off + kbase + e]];
-                } else {
-                    b[e] = (kbase + e) < it.ncols ? S[it.col_off + kbase + e] : acc_t(0);
-                }
-            }
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                if (rt < nrt) {
-                    const int rr = min(16, rows - 16 * rt);
-                    if (li < rr) {
-                        u32x4 raw = *reinterpret_cast<const u32x4*>(slab + 16 * rt * TK +
-                                                                     (rr * lg + li) * EPL);
-                        if constexpr (std::is_same<TS, float>::value) {
-                            f32x4 a = __builtin_bit_cast(f32x4, raw);
-                            acc[rt] += a.x * b[0] + a.y * b[1] + a.z * b[2] + a.w * b[3];
-                        } else if constexpr (std::is_same<TS, double>::value) {
-                            typedef double d2 __attribute__((ext_vector_type(2)));
-                            d2 a = __builtin_bit_cast(d2, raw);
-                            acc[rt] += a.x * b[0] + a.y * b[1];
-                        } else {
-                            uint16_t h[8];
-                            std::memcpy(h, &raw, 16);
-                            acc_t s = 0;
-#pragma unroll
-                            for (int e = 0; e < 8; e++) s += bf2f(h[e]) * b[e];
-                            acc[rt] += s;
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++) {
-            acc_t v = acc[rt];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (lg == 0 && rt < nrt) red[wave][16 * rt + li][0] = v;
-        }
+    switch ((rows + 15) >> 4) {
+    case 1:
+        panel_body<TS, W, LEAF, 1>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
+        break;
+    case 2:
+        panel_body<TS, W, LEAF, 2>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
+        break;
+    case 3:
+        panel_body<TS, W, LEAF, 3>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
+        break;
+    default:
+        panel_body<TS, W, LEAF, 4>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
+        break;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < rows * W; t += kWaves * 64) {
@@ -490,6 +597,50 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
+// Work items are generated in cluster-tree order, i.e. spatially coherent:
+// neighbouring leaves share most near-field source rows and the T rows of the
+// low-rank blocks above them.  Workgroups are handed to the XCDs round-robin
+// (workgroup i -> XCD i % kXcds), each XCD with its own L2, so the item list is
+// cut into kXcds contiguous chunks of equal work and interleaved: every XCD
+// then walks one compact region of space in tree order.  HMX_ITEM_ORDER=lpt
+// selects largest-first order instead (better tail balance, no locality).
+constexpr int kXcds = 8;
+
+void order_items_for_xcds(std::vector<Item>& items) {
+    if (items.size() < 2 * kXcds) return;
+    auto work = [](const Item& it) { return int64_t(it.ktiles) * it.rows * 64 + 4096; };
+    const char* env = std::getenv("HMX_ITEM_ORDER");
+    if (env && std::string(env) == "lpt") {
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const Item& a, const Item& b) { return work(a) > work(b); });
+        return;
+    }
+    if (env && std::string(env) == "tree") return;
+    int64_t total = 0;
+    for (const Item& it : items) total += work(it);
+    std::vector<std::vector<Item>> chunk(kXcds);
+    int64_t acc = 0;
+    for (const Item& it : items) {
+        const int c = static_cast<int>(std::min<int64_t>(kXcds - 1, acc * kXcds / total));
+        chunk[c].push_back(it);
+        acc += work(it);
+    }
+    std::vector<size_t> pos(kXcds, 0);
+    std::vector<Item> out;
+    out.reserve(items.size());
+    for (size_t p = 0; out.size() < items.size(); p++) {
+        int c = static_cast<int>(p % kXcds);
+        if (pos[c] == chunk[c].size()) {  // exhausted: borrow from the fullest chunk
+            size_t best = 0;
+            for (int q = 0; q < kXcds; q++)
+                if (chunk[q].size() - pos[q] > chunk[best].size() - pos[best]) best = q;
+            c = static_cast<int>(best);
+        }
+        out.push_back(chunk[c][pos[c]++]);
+    }
+    items.swap(out);
+}
+
 }  // namespace
 
 void device_destroy(DeviceOp* op) {
@@ -519,14 +670,14 @@ int64_t device_index_bytes(const DeviceOp* op) {
 }
 
 // Build the device operator for one storage precision.
-void device_build(HMatrix& h, int storage, const ClusterTree& T) {
+void device_build(HMatrix& h, int storage) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         throw Error(HMX_ERR_HIP, "no HIP device available");
     if (h.device < 0 || h.device >= ndev) throw Error(HMX_ERR_ARG, "invalid device ordinal");
     if (h.m >= (int64_t(1) << 31) || h.n + h.R + 64 >= (int64_t(1) << 31))
         throw Error(HMX_ERR_UNSUPPORTED, "matrix too large for 32-bit device indices");
-    if (h.src.kernel_id != HMX_KERNEL_CALLBACK && (T.dims != 1))
+    if (h.src.kernel_id != HMX_KERNEL_CALLBACK && (h.target_dims != 1 || h.source_dims != 1))
         throw Error(HMX_ERR_UNSUPPORTED, "built-in kernels need dims == 1");
     DeviceGuard guard(h.device);
     auto op = std::make_unique<DeviceOp>();
@@ -710,12 +861,8 @@ void device_build(HMatrix& h, int storage, const ClusterTree& T) {
                     (void*)dgen, (void*)dvrow, (void*)dA, (void*)dB})
         if (p) (void)hipFree(p);
 
-    // Dispatch order: largest panels first (longest-processing-time-first).
-    auto by_work = [](const Item& a, const Item& b) {
-        return int64_t(a.ktiles) * a.rows > int64_t(b.ktiles) * b.rows;
-    };
-    std::stable_sort(itemsA.begin(), itemsA.end(), by_work);
-    std::stable_sort(itemsB.begin(), itemsB.end(), by_work);
+    order_items_for_xcds(itemsA);
+    order_items_for_xcds(itemsB);
     op->itemsA = upload(itemsA);
     op->nA = static_cast<int64_t>(itemsA.size());
     op->itemsB = upload(itemsB);
@@ -728,11 +875,12 @@ void device_build(HMatrix& h, int storage, const ClusterTree& T) {
 // ---- product ---------------------------------------------------------------------
 namespace {
 
+int64_t s_rows(const DeviceOp& op) { return op.n + op.R + 1 + tk_of(op.storage); }
+
 Scratch& get_scratch(DeviceOp& op, hipStream_t stream, int W) {
     std::lock_guard<std::mutex> g(op.mu);
     Scratch& s = op.scratch[static_cast<void*>(stream)];
-    const int TK = tk_of(op.storage);
-    const int64_t srows = op.n + op.R + 1 + TK;
+    const int64_t srows = s_rows(op);
     size_t sel = op.storage == HMX_F64 ? 8 : (W == 16 && op.storage == HMX_BF16 ? 2 : 4);
     size_t pel = op.storage == HMX_F64 ? 8 : 4;
     void*& Sp = W == 16 ? s.S16 : s.S1;
@@ -772,6 +920,10 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
     s_t* S = static_cast<s_t*>(W == 16 ? sc.S16 : sc.S1);
     acc_t* P = static_cast<acc_t*>(W == 16 ? sc.P16 : sc.P1);
     const TS* panel = static_cast<const TS*>(op.panel);
+    const int64_t s_total = s_rows(op) * W * static_cast<int64_t>(sizeof(s_t));
+    if (s_total >= (int64_t(1) << 32))
+        throw Error(HMX_ERR_UNSUPPORTED, "source/T staging array exceeds 4 GiB (32-bit buffer offsets)");
+    const uint32_t s_bytes = static_cast<uint32_t>(s_total);
     for (int rhs0 = 0; rhs0 < nrhs; rhs0 += W) {
         {
             StageTimer t(op, st, 0);
@@ -781,7 +933,8 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
         if (op.nA) {
             StageTimer t(op, st, 1);
             hipLaunchKernelGGL((k_panel<TS, W, false, TIO>), dim3(op.nA), dim3(kWaves * 64), 0, st,
-                               op.itemsA, panel, nullptr, S, op.n, P, nullptr, nullptr, 0, 0, 0);
+                               op.itemsA, panel, nullptr, S, op.n, s_bytes, P, nullptr, nullptr, 0,
+                               0, 0);
         }
         if (op.nG) {
             StageTimer t(op, st, 2);
@@ -791,8 +944,8 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
         if (op.nB) {
             StageTimer t(op, st, 3);
             hipLaunchKernelGGL((k_panel<TS, W, true, TIO>), dim3(op.nB), dim3(kWaves * 64), 0, st,
-                               op.itemsB, panel, op.colidx, S, op.n, P, op.tmap0, Y, ldy, rhs0,
-                               nrhs);
+                               op.itemsB, panel, op.colidx, S, op.n, s_bytes, P, op.tmap0, Y, ldy,
+                               rhs0, nrhs);
         }
     }
     HIPCHECK(hipGetLastError());

@@ -130,6 +130,7 @@ struct HMatrix {
     std::vector<int64_t> target_leaf_start;  // 0-based start rows of target leaves (+ end)
     double build_seconds = 0.0;
     int32_t device = 0;
+    int64_t target_dims = 1, source_dims = 1;
     std::unique_ptr<DeviceOp, DeviceOpDeleter> dev[3];
     void ensure_dense_payload();
 };
@@ -143,7 +144,7 @@ void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f)
 int default_threads();
 
 // ---- device side (device.hip) -------------------------------------------------
-void device_build(HMatrix& h, int storage, const ClusterTree& T);
+void device_build(HMatrix& h, int storage);
 void device_matvec(HMatrix& h, int storage, const void* X, int64_t ldx, void* Y, int64_t ldy,
                    int nrhs, int io_dtype, void* stream);
 int64_t device_panel_bytes(const DeviceOp* op);

@@ -122,6 +122,8 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
     h->n = static_cast<int64_t>(S.index_map.size());
     h->tmap = T.index_map;
     h->smap = S.index_map;
+    h->target_dims = T.dims;
+    h->source_dims = S.dims;
 
     auto bt = build_block_tree(T, S, eta);  // hmatrix.jl:74
     merge_dense(*bt);                       // :75

@@ -47,6 +47,8 @@ SIGNATURES = {
     "hmx_hmatrix_build": (C.c_int, [vp, vp, C.c_int32, f64p, f64p, BLOCK_FN, vp, C.c_double,
                                     C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(vp)]),
+    "hmx_hmatrix_materialize": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "hmx_hmatrix_release": (C.c_int, [vp, C.c_int32]),
     "hmx_hmatrix_sizes": (C.c_int, [vp, i64p]),
     "hmx_hmatrix_export_reference_layout": (C.c_int, [vp, i64p, i64p, i64p, i64p, i64p, f64p,
                                                       f64p, f64p]),

@@ -180,6 +180,23 @@ class HMatrix:
     def shape(self):
         return (self.m, self.n)
 
+    def materialize(self, storage, device=None):
+        """Build (or rebuild) the device operator for the given precision(s)."""
+        mask = 0
+        for st in ([storage] if isinstance(storage, str) else storage):
+            mask |= 1 << STORAGE[st.lower()]
+        dev = self.device if device is None else int(device)
+        L.check(L.lib.hmx_hmatrix_materialize(self._h, mask, dev))
+        self.device = dev
+        self.storage_mask |= mask
+
+    def release(self, storage):
+        mask = 0
+        for st in ([storage] if isinstance(storage, str) else storage):
+            mask |= 1 << STORAGE[st.lower()]
+        L.check(L.lib.hmx_hmatrix_release(self._h, mask))
+        self.storage_mask &= ~mask
+
     def info(self):
         o = np.zeros(16)
         L.check(L.lib.hmx_hmatrix_info(self._h, L.ptr_f64(o)))

@@ -119,6 +119,11 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
                       const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
                       void* ctx, double eta, double eps, int32_t svd_recompress,
                       int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out);
+/* (Re)build the device operator of every precision in storage_mask on `device`
+ * from the host-side H-matrix (the reference's kernel_array uploads,
+ * hmatrix.jl:104,133-147), replacing an existing one; release frees it. */
+int hmx_hmatrix_materialize(hmx_hmatrix* h, int32_t storage_mask, int32_t device);
+int hmx_hmatrix_release(hmx_hmatrix* h, int32_t storage_mask);
 /* sizes[0..7] = m, n, n_dense, n_lowrank, R (= length(Vx_buffer)), length of
  * dense_blocks, U_matrices, V_matrices in the flattened layout. */
 int hmx_hmatrix_sizes(const hmx_hmatrix* h, int64_t* sizes);

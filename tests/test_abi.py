@@ -1,0 +1,126 @@
+"""The C ABI (include/hmx.h): every declared entry point is exported by the
+in-tree libhmx.so, status codes and hmx_last_error follow the documented
+conventions.  No GPU is touched here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hmatrixgpu as H
+from hmatrixgpu import _lib as L
+from helpers import uniform_points
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "hmx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hmx_[a-z0-9_]+)\s*\(", src)) - {"hmx_block_fn", "hmx_vec_fn"})
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(L.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 28
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding declares a signature for each of them
+    assert set(names) <= set(L.SIGNATURES)
+
+
+def test_library_is_the_in_tree_build():
+    assert os.path.realpath(L.LIB_PATH).startswith(os.path.realpath(ROOT))
+
+
+def test_version():
+    a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+    L.check(L.lib.hmx_version(C.byref(a), C.byref(b), C.byref(c)))
+    assert (a.value, b.value) == (0, 1)
+
+
+def test_dims_error_matches_reference():
+    pts = uniform_points(10, 1)
+    with pytest.raises(L.HMXError) as e:
+        H.ClusterTree(pts, max_points_per_leaf=4, dims=0)
+    assert e.value.code == 2  # HMX_ERR_DIMS
+    assert "dims" in str(e.value)
+
+
+def test_empty_cluster_error():
+    pts = np.zeros((3, 9))
+    with pytest.raises(L.HMXError) as e:
+        H.ClusterTree(pts, max_points_per_leaf=4)
+    assert e.value.code == 3  # HMX_ERR_EMPTY_CLUSTER
+
+
+def test_null_arguments_rejected():
+    out = L.vp()
+    st = L.lib.hmx_tree_build(None, 10, 4, 1, C.byref(out))
+    assert st == 1 and not out.value
+    assert b"empty" in L.lib.hmx_last_error()
+    st = L.lib.hmx_blocktree_build(None, None, 1.5, C.byref(out))
+    assert st == 1
+
+
+def test_matvec_without_device_storage_is_a_state_error():
+    pts = uniform_points(500, 2)
+    X = H.ClusterTree(pts, max_points_per_leaf=32)
+    hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=1e-6, storage=())
+    st = L.lib.hmx_matvec(hm._h, L.HMX_F32, C.c_void_p(8), 500, C.c_void_p(8), 500, 16,
+                          L.HMX_F32, None)
+    assert st == 7  # HMX_ERR_STATE: nothing materialised, no GPU call made
+    assert b"storage" in L.lib.hmx_last_error()
+
+
+def test_callback_errors_propagate():
+    pts = uniform_points(300, 3)
+    X = H.ClusterTree(pts, max_points_per_leaf=16)
+
+    class Broken:
+        shape = (300, 300)
+
+        def __getitem__(self, idx):
+            raise KeyError("broken entries")
+
+    with pytest.raises(KeyError):
+        H.HMatrix(Broken(), X, X, eta=1.5, eps=1e-6, storage=())
+
+
+def test_builtin_kernel_requires_scalar_dims():
+    pts = uniform_points(200, 4)
+    X = H.ClusterTree(pts, max_points_per_leaf=16)
+    Y = H.ClusterTree(pts, max_points_per_leaf=16, dims=3)
+    with pytest.raises(L.HMXError) as e:
+        H.HMatrix(H.LaplaceKernel(pts), X, Y, eta=1.5, eps=1e-6, storage=())
+    assert e.value.code == 8
+
+
+def test_info_keys_mirror_reference():
+    pts = uniform_points(2000, 5)
+    X = H.ClusterTree(pts, max_points_per_leaf=32)
+    hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=1e-6, storage=())
+    d = H.info(hm)
+    for k in ("data_type", "size", "leaves", "admissible_leaves", "full_leaves", "min_rank",
+              "max_rank", "min_dense_size", "max_dense_size", "compression_ratio"):
+        assert k in d  # hmatrix.jl:289-330
+    assert d["leaves"] == d["admissible_leaves"] + d["full_leaves"]
+    t = H.info(X)
+    assert set(t) == {"depth", "max_points", "min_points"}  # tree.jl:153-187
+
+
+def test_hmatrixcpu_fields():
+    """flatten=False -> HMatrixCPU with per-block matrices (hmatrix.jl:18-27)."""
+    pts = uniform_points(1500, 6)
+    X = H.ClusterTree(pts, max_points_per_leaf=32)
+    hc = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=1e-6, flatten=False, storage=())
+    assert isinstance(hc, H.HMatrixCPU)
+    assert len(hc.dense_blocks) == len(hc.dense_block_indices)
+    assert len(hc.U_matrices) == len(hc.V_matrices) == len(hc.approx_block_indices)
+    for (r0, r1, c0, c1), D in zip(hc.dense_block_indices, hc.dense_blocks):
+        assert D.shape == (r1 - r0 + 1, c1 - c0 + 1)
+    for (r0, r1, c0, c1), U, V in zip(hc.approx_block_indices, hc.U_matrices, hc.V_matrices):
+        assert U.shape[0] == r1 - r0 + 1 and V.shape[1] == c1 - c0 + 1
+        assert U.shape[1] == V.shape[0]

@@ -1,0 +1,161 @@
+"""Product host builders (libhmx, C++) against the oracle, bit for bit.
+
+The cluster tree, permutation, block tree, block lists and the whole flattened
+layout (index tables and FP64 payloads) of the product must equal the oracle's
+(the restatement of tree.jl, block.jl, aca.jl, hmatrix.jl) exactly; the
+product's low-rank arithmetic follows the same operation order, so even the
+U/V payloads match to the bit.  CPU only: the builders run on the host.
+"""
+import numpy as np
+import pytest
+
+import hmatrixgpu as H
+import oracle_lib as O
+from helpers import uniform_points
+
+
+def _same_tree(pts_3n, leaf, dims=1):
+    t_o = O.Tree(pts_3n.T, leaf, dims)
+    t_p = H.ClusterTree(pts_3n, max_points_per_leaf=leaf, dims=dims)
+    assert np.array_equal(t_o.index_map, t_p.index_map)
+    assert np.array_equal(t_o.start, t_p.node_start)
+    assert np.array_equal(t_o.end, t_p.node_end)
+    assert np.array_equal(t_o.center, t_p.node_center)  # bitwise
+    assert np.array_equal(t_o.radius, t_p.node_radius)
+    assert np.array_equal(t_o.left, t_p.node_left)
+    assert np.array_equal(t_o.right, t_p.node_right)
+    assert np.array_equal(t_o.is_leaf, t_p.node_is_leaf)
+    return t_o, t_p
+
+
+@pytest.mark.parametrize("n,leaf,dims,seed", [(1000, 10, 1, 1), (4096, 32, 1, 2),
+                                              (20000, 64, 1, 3), (700, 16, 3, 4)])
+def test_cluster_tree_bit_exact(n, leaf, dims, seed):
+    _same_tree(uniform_points(n, seed), leaf, dims)
+
+
+def test_cluster_tree_planar_and_lattice():
+    # ties everywhere: a regular lattice with duplicated split coordinates
+    g = np.stack(np.meshgrid(np.arange(12.0), np.arange(10.0), np.arange(3.0)), 0).reshape(3, -1)
+    _same_tree(g, 8)
+    pts = uniform_points(3000, 5)
+    pts[2] = 0.0
+    _same_tree(pts, 10)
+
+
+@pytest.mark.parametrize("eta", [0.5, 1.5, 2.0])
+def test_block_tree_bit_exact(eta):
+    pts = uniform_points(3000, 6)
+    t_o = O.Tree(pts.T, 16)
+    s_o = O.Tree(pts.T, 8)
+    t_p = H.ClusterTree(pts, max_points_per_leaf=16)
+    s_p = H.ClusterTree(pts, max_points_per_leaf=8)
+    for merge in (False, True):
+        b_o = O.BlockTree(t_o, s_o, eta, merge=merge)
+        b_p = H.BlockTree(t_p, s_p, eta=eta)
+        if merge:
+            H.merge_dense_matrices_(b_p.root)
+        assert np.array_equal(b_o.tnode, b_p.node_target)
+        assert np.array_equal(b_o.snode, b_p.node_source)
+        assert np.array_equal(b_o.adm, b_p.node_admissible)
+        assert np.array_equal(b_o.dense, b_p.node_dense)
+        assert np.array_equal(b_o.left, b_p.node_left)
+        assert np.array_equal(b_o.right, b_p.node_right)
+        d_o, a_o = b_o.traverse()
+        d_p, a_p = b_p.lists()
+        assert np.array_equal(d_o, d_p) and np.array_equal(a_o, a_p)
+    # the object view mirrors the reference structs
+    dl, al = H.traverse(b_p)
+    assert all(isinstance(t, H.ClusterNode) for t, _ in dl + al)
+
+
+def _layouts_equal(Ho, Hp):
+    lay = Hp.layout(True)
+    assert np.array_equal(lay["target_index_map"], Ho.target_index_map)
+    assert np.array_equal(lay["source_index_map"], Ho.source_index_map)
+    assert np.array_equal(lay["dense_block_indices"], Ho.dense_block_indices.T)
+    assert np.array_equal(lay["U_block_indices"], Ho.U_block_indices.T)
+    assert np.array_equal(lay["V_block_indices"], Ho.V_block_indices.T)
+    assert np.array_equal(lay["dense_blocks"], Ho.dense_blocks)
+    assert np.array_equal(lay["U_matrices"], Ho.U_matrices)
+    assert np.array_equal(lay["V_matrices"], Ho.V_matrices)
+
+
+def test_hmatrix_layout_config1_bit_exact():
+    """Config 1 (N=4096, leaf 32, eta 2, tol 1e-6) and config 2's parameters at a
+    CPU-friendly size: identical flattened layouts."""
+    for n, leaf in ((4096, 32), (8192, 64)):
+        pts = uniform_points(n, 1)
+        t_o, t_p = _same_tree(pts, leaf)
+        K = O.make_kernel(O.K_LAPLACE, xt=pts.T, xs=pts.T)
+        Ho = O.HMatrix(K, t_o, t_o, 2.0, 1e-6)
+        Hp = H.HMatrix(H.LaplaceKernel(pts), t_p, t_p, eta=2.0, eps=1e-6, storage=())
+        _layouts_equal(Ho, Hp)
+        info = Hp.info()
+        assert info["admissible_leaves"] == Ho.na and info["full_leaves"] == Ho.nd
+
+
+def test_hmatrix_callback_and_builtin_agree():
+    """test_hmatrix.jl circle through a host callback (any matrix-like K) and
+    through the native kernel: same layout as the oracle."""
+    n = 1000
+    ang = np.arange(1, n + 1) * 2 * np.pi / n
+    pts = np.stack([np.sin(ang), np.cos(ang), np.zeros(n)])
+    t_o, t_p = _same_tree(pts, 64)
+    K = O.make_kernel(O.K_LOG2D, xt=pts.T, xs=pts.T)
+    Ho = O.HMatrix(K, t_o, t_o, 1.5, 1e-6)
+    kern = H.Log2DKernel(pts)
+    Hp = H.HMatrix(kern, t_p, t_p, eta=1.5, eps=1e-6, storage=())
+    lay = Hp.layout(True)
+    for k in ("dense_block_indices", "U_block_indices", "V_block_indices"):
+        assert np.array_equal(lay[k], getattr(Ho, k).T)
+    # log() may differ in the last bit between libm and numpy, nothing else
+    assert np.allclose(lay["dense_blocks"], Ho.dense_blocks, rtol=1e-15, atol=0)
+
+    class Dense:
+        def __init__(self, M):
+            self.M, self.shape = M, M.shape
+
+        def __getitem__(self, idx):
+            return self.M[idx]
+
+    Kd = O.kernel_matrix(K)
+    Hc = H.HMatrix(Dense(Kd), t_p, t_p, eta=1.5, eps=1e-6, storage=())
+    _layouts_equal(Ho, Hc)
+
+
+def test_aca_and_recompress_via_abi_match_oracle():
+    N = 100
+    i = np.arange(1, N + 1)[:, None]
+    j = np.arange(1, N + 1)[None, :]
+    with np.errstate(divide="ignore"):
+        K = np.where(i != j, N / (i - j) ** 2.0, 0.0)
+    B = K[N - 30:, :30]
+    U, V = H.ACA_plus(30, 30, lambda I: B[I - 1, :], lambda J: B[:, J - 1], 1e-7)
+    Uo, Vo = O.aca_plus(O.make_kernel(O.K_MATRIX, mat=B), np.arange(30), np.arange(30), 1e-7)
+    assert np.array_equal(U, Uo) and np.array_equal(V, Vo)
+    U2, V2 = H.SVD_recompress(U, V, 1e-7)
+    U2o, V2o = O.svd_recompress(Uo, Vo, 1e-7)
+    assert np.array_equal(U2, U2o) and np.array_equal(V2, V2o)
+
+
+def test_vector_kernel_dims3_callback():
+    n = 300
+    ang = np.arange(1, n + 1) * 2 * np.pi / n
+    pts = np.stack([np.sin(ang), np.cos(ang), np.zeros(n)])
+    t_o = O.Tree(pts.T, 64)
+    s_o = O.Tree(pts.T, 64, dims=3)
+    t_p = H.ClusterTree(pts, max_points_per_leaf=64)
+    s_p = H.ClusterTree(pts, max_points_per_leaf=64, dims=3)
+    K = O.make_kernel(O.K_VEC3, xt=pts.T, xs=pts.T, m=n, n=3 * n)
+    Ho = O.HMatrix(K, t_o, s_o, 1.5, 1e-6)
+    Kd = O.kernel_matrix(K)
+
+    class Dense:
+        shape = Kd.shape
+
+        def __getitem__(self, idx):
+            return Kd[idx]
+
+    Hp = H.HMatrix(Dense(), t_p, s_p, eta=1.5, eps=1e-6, storage=())
+    _layouts_equal(Ho, Hp)

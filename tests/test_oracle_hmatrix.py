@@ -1,0 +1,134 @@
+"""Pins for the oracle's H-matrix build, flattened layout and CPU product
+(hmatrix.jl:72-217, mult.jl:6-32).
+
+Restated reference tests:
+  test_hmatrix.jl:10-55   unit circle N=1000, -log(d)/2pi, leaf 64, eta 1.5,
+                          eps 1e-6: compression > 3, no NaN, |Hx - Kx|_2 <= 1e-5,
+                          max(V_off + ncols) == length(V_data)
+  test_hmatrix_vector.jl  vector kernel R[j]/r^2 with a dims=3 source tree
+The layout semantics of orc_mul_layout are pinned by an independent numpy
+evaluation of the exported flattened arrays.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def _circle(n=1000):
+    ang = np.arange(1, n + 1) * 2 * np.pi / n
+    return np.stack([np.sin(ang), np.cos(ang), np.zeros(n)], 1)
+
+
+@pytest.fixture(scope="module")
+def circle():
+    pts = _circle()
+    K = O.make_kernel(O.K_LOG2D, xt=pts, xs=pts)
+    T = O.Tree(pts, 64)
+    H = O.HMatrix(K, T, T, 1.5, 1e-6)
+    return pts, K, T, H
+
+
+def numpy_layout_product(H, x):
+    """Independent evaluation of the flattened layout (hmatrix.jl:95-147)."""
+    xo = x[H.source_index_map - 1]
+    y = np.zeros(len(H.target_index_map))
+    for r0, r1, c0, c1, off in H.dense_block_indices:
+        m, n = r1 - r0 + 1, c1 - c0 + 1
+        D = H.dense_blocks[off:off + m * n].reshape(m, n)
+        y[r0 - 1:r1] += D @ xo[c0 - 1:c1]
+    vx = np.zeros(len(H.V_block_indices))
+    for t, (c0, c1, off) in enumerate(H.V_block_indices):
+        vx[t] = H.V_matrices[off:off + c1 - c0 + 1] @ xo[c0 - 1:c1]
+    for r0, r1, v0, v1, off in H.U_block_indices:
+        m, k = r1 - r0 + 1, v1 - v0 + 1
+        U = H.U_matrices[off:off + m * k].reshape(m, k)
+        y[r0 - 1:r1] += U @ vx[v0 - 1:v1]
+    out = np.zeros_like(y)
+    out[H.target_index_map - 1] = y
+    return out
+
+
+def test_reference_circle(circle):
+    pts, K, T, H = circle
+    n = len(pts)
+    nnz = len(H.dense_blocks) + len(H.U_matrices) + len(H.V_matrices)
+    assert n * n / nnz > 3  # test_hmatrix.jl:30-32
+    for a in (H.dense_blocks, H.U_matrices, H.V_matrices):
+        assert not np.any(np.isnan(a))
+    x = np.random.default_rng(1).random(n)
+    Kd = O.kernel_matrix(K)
+    y = H.matvec(x)
+    assert np.linalg.norm(y - Kd @ x) <= 1e-5  # test_hmatrix.jl:49-50
+    # test_hmatrix.jl:52-55: the last V row ends at the end of V_data
+    vi = H.V_block_indices
+    assert np.max(vi[:, 2] + vi[:, 1] - vi[:, 0] + 1) == len(H.V_matrices)
+
+
+def test_layout_product_matches_numpy(circle):
+    pts, K, T, H = circle
+    x = np.random.default_rng(2).standard_normal(len(pts))
+    assert np.allclose(H.matvec(x), numpy_layout_product(H, x), rtol=1e-13, atol=1e-13)
+
+
+def test_layout_tables_are_consistent(circle):
+    _, _, _, H = circle
+    di, ui, vi = H.dense_block_indices, H.U_block_indices, H.V_block_indices
+    sizes = (di[:, 1] - di[:, 0] + 1) * (di[:, 3] - di[:, 2] + 1)
+    assert np.array_equal(di[:, 4], np.concatenate([[0], np.cumsum(sizes)[:-1]]))
+    ranks = ui[:, 3] - ui[:, 2] + 1
+    assert np.array_equal(ui[:, 2], np.concatenate([[1], 1 + np.cumsum(ranks)[:-1]]))
+    usz = (ui[:, 1] - ui[:, 0] + 1) * ranks
+    assert np.array_equal(ui[:, 4], np.concatenate([[0], np.cumsum(usz)[:-1]]))
+    ncols = vi[:, 1] - vi[:, 0] + 1
+    assert np.array_equal(vi[:, 2], np.concatenate([[0], np.cumsum(ncols)[:-1]]))
+    # every low-rank block passed the storage test (m + n) k < m n
+    m = ui[:, 1] - ui[:, 0] + 1
+    c0 = np.array([vi[v - 1, 0] for v in ui[:, 2]])
+    c1 = np.array([vi[v - 1, 1] for v in ui[:, 2]])
+    n = c1 - c0 + 1
+    assert np.all(m * ranks + ranks * n < m * n)
+
+
+def test_config1_accuracy_and_fallback_order():
+    """N=4096 uniform [0,1)^3, 1/r, leaf 32, eta 2, tol 1e-6 (config 1)."""
+    pts = np.random.default_rng(1).random((4096, 3))
+    K = O.make_kernel(O.K_LAPLACE, xt=pts, xs=pts)
+    T = O.Tree(pts, 32)
+    H = O.HMatrix(K, T, T, 2.0, 1e-6)
+    x = np.random.default_rng(3).standard_normal(4096)
+    y = H.matvec(x)
+    yd = O.dense_matvec(K, x)
+    assert np.linalg.norm(y - yd) / np.linalg.norm(yd) < 1e-7
+    # storage fallbacks are appended after the direct list, in approx order:
+    # the tail of the dense list consists of admissible (well separated) blocks
+    bt = O.BlockTree(T, T, 2.0, merge=True)
+    d, a = bt.traverse()
+    nd = len(d)
+    assert H.nd >= nd
+    direct_ranges = [(T.start[t], T.end[t] - 1, T.start[s], T.end[s] - 1) for t, s in d]
+    assert [tuple(r[:4]) for r in H.dense_block_indices[:nd]] == direct_ranges
+    approx_ranges = [(T.start[t], T.end[t] - 1, T.start[s], T.end[s] - 1) for t, s in a]
+    fb = [tuple(r[:4]) for r in H.dense_block_indices[nd:]]
+    lr = [tuple(r[:2]) for r in H.U_block_indices]
+    # fallback blocks + low-rank blocks partition the approx list, each in order
+    assert set(fb) <= set(approx_ranges)
+    assert len(fb) + len(lr) == len(approx_ranges)
+    pos = [approx_ranges.index(f) for f in fb]
+    assert pos == sorted(pos)
+
+
+def test_vector_kernel_dims3():
+    """test_hmatrix_vector.jl:12-54: N x 3N kernel R[j]/r^2, dims=3 sources."""
+    n = 300
+    pts = _circle(n)
+    K = O.make_kernel(O.K_VEC3, xt=pts, xs=pts, m=n, n=3 * n)
+    T = O.Tree(pts, 64)
+    S = O.Tree(pts, 64, dims=3)
+    H = O.HMatrix(K, T, S, 1.5, 1e-6)
+    assert sorted(H.source_index_map.tolist()) == list(range(1, 3 * n + 1))
+    x = np.random.default_rng(4).random(3 * n)
+    Kd = O.kernel_matrix(K)
+    assert np.linalg.norm(H.matvec(x) - Kd @ x) <= 1e-5
+    vi = H.V_block_indices
+    assert np.max(vi[:, 2] + vi[:, 1] - vi[:, 0] + 1) == len(H.V_matrices)
