@@ -42,7 +42,62 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-gb", type=float, default=4.0,
                    help="FP64 payload bytes of the CPU-baseline sample")
+    p.add_argument("--profile-run", action="store_true",
+                   help="workload for rocprofv3: a 4 GiB copy and a 4 GiB fill (counter "
+                        "calibration), then 5 FP32 16-RHS products; prints no JSON")
     return p.parse_args()
+
+
+PMC_FILE = os.path.join(ROOT, "profiles", "leaf_panels_fp32_16rhs_n262144.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the leaf-panel kernel from the committed rocprofv3
+    counter summary: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the same counter read for a copy / fill of known size."""
+    if not os.path.exists(PMC_FILE):
+        return None, None
+    vals = {}
+    for line in open(PMC_FILE):
+        if line.startswith("    ") and "=" in line:
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    try:
+        rscale = vals["copy_read_bytes"] / (vals["copy_fetch_kib"] * 1024.0)
+        wscale = vals["fill_write_bytes"] / (vals["fill_write_kib"] * 1024.0)
+        traffic = (vals["leaf_fetch_kib"] * 1024.0 * rscale + vals["leaf_write_kib"] * 1024.0 * wscale)
+        return traffic, {"fetch_scale": rscale, "write_scale": wscale, "source": os.path.relpath(PMC_FILE, ROOT)}
+    except KeyError:
+        return None, None
+
+
+def profile_run(args):
+    import numpy as np
+    import torch
+
+    import hmatrixgpu as H
+    dev = torch.device("cuda", 0)
+    a = torch.empty(2 ** 29, dtype=torch.float64, device=dev)  # 4 GiB
+    a.fill_(1.0)  # calibration: 4 GiB written
+    b = torch.empty_like(a)
+    b.copy_(a)  # calibration: 4 GiB read + 4 GiB written
+    torch.cuda.synchronize()
+    log("calibration copy/fill of 4 GiB done")
+    del a, b
+    n = args.n
+    pts = np.random.default_rng(3).random((n, 3)).T.copy()
+    X = H.ClusterTree(pts, max_points_per_leaf=args.leaf)
+    hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=args.eta, eps=args.tol, storage=("f32",))
+    log("built: %s" % json.dumps(hm.info()))
+    Xg = torch.randn((args.nrhs, n), dtype=torch.float32, device=dev).t()
+    Y = torch.empty((args.nrhs, n), dtype=torch.float32, device=dev).t()
+    for _ in range(5):
+        H.mul_(Y, hm, Xg, storage="f32")
+    torch.cuda.synchronize()
+    log("5 FP32 %d-RHS products done" % args.nrhs)
 
 
 def dense_reference_rows(torch, pts, rows, X64, dev):
@@ -127,6 +182,9 @@ def main():
               "by target rows in the design, but no multi-GPU path is built); run with --gpus 1",
               flush=True)
         sys.exit(2)
+    if args.profile_run:
+        profile_run(args)
+        return
     import numpy as np
     import torch
 
@@ -199,6 +257,7 @@ def main():
     leaf_bytes = 4 * (nnzD + nnzU) + n * nrhs * 4  # dominant kernel: leaf panels + Y
     achieved = leaf_bytes / (stage_avg[3] * 1e-3) / 1e9 if stage_avg[3] > 0 else 0.0
     value = nrhs / t32
+    traffic, traffic_cal = measured_traffic() if n == 262144 and nrhs == 16 else (None, None)
     res = {
         "metric": METRIC, "value": value, "unit": "H-matvecs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t32 * 1e3,
@@ -211,8 +270,9 @@ def main():
                    "storage": "fp32", "parallelism": "single-gpu"},
         "hbm_gbs": b32 / t32 / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_panel leaf panels (near field + U*T), FP32, 16 RHS",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_calibration": traffic_cal,
+                     "kernel": "k_leaf: leaf panels (near field + U*T), FP32, 16 RHS",
                      "kernel_ms": stage_avg[3], "kernel_alg_bytes": leaf_bytes},
         "stages_ms": {"gather": stage_avg[0], "far_field_VX": stage_avg[1],
                       "partial_reduce": stage_avg[2], "leaf_panels": stage_avg[3]},

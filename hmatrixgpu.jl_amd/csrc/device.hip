@@ -150,6 +150,10 @@ constexpr int kWaves = 4;  // waves per workgroup in the panel kernels
 // the absent rows of a partial row tile, the padding of the last k-tile and the
 // prefetch past the end of a panel without branches.
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer descriptor
+// Cache policy of the streamed panel loads: non-temporal, so that the matrix
+// stream (read once per product) does not evict the reused source / T rows
+// from L2 (measured: 3-10% faster than the default policy at N = 262,144).
+constexpr int kPanelPolicy = 2;
 constexpr uint32_t kOOB = 0xffffffffu;
 
 __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
@@ -203,25 +207,27 @@ __device__ inline bf16s buf_load<bf16s>(__amdgpu_buffer_rsrc_t R, uint32_t off) 
     return bf16s{__builtin_amdgcn_raw_buffer_load_b16(R, off, 0, 0)};
 }
 
-// One wave's share of a panel: k-tiles wave, wave + kWaves, ... accumulated
-// into red[row][col] (this wave's slice).  Loads are issued one k-tile ahead
-// (A and B) and two k-tiles ahead (column indices).
-template <typename TS, int W, bool LEAF, int NRT>
-__device__ __forceinline__ void panel_body(const Item& it, const TS* panel, const int32_t* colidx,
+// One wave's share of a panel: k-tiles kt0, kt0 + kstep, ... of a `rows` x
+// (ktiles * TK) panel at pdata.  B rows come from S: through the column index
+// array (LEAF) or as the contiguous rows col_off + k masked beyond ncols.  The
+// accumulated results are handed to emit(row, col, value), row < 64.  Loads are
+// issued one k-tile ahead (A and B) and two ahead (column indices).  CP is the
+// cache policy of the streamed A loads.
+template <typename TS, int W, bool LEAF, int NRT, int CP, typename Emit>
+__device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles, int64_t col_off,
+                                           int ncols, const int32_t* colidx,
                                            const typename SType<TS, W>::type* S, uint32_t s_bytes,
-                                           int wave, int lane,
-                                           typename Store<TS>::acc (*red)[W]) {
+                                           int kt0, int kstep, int lane, Emit&& emit) {
     using acc_t = typename Store<TS>::acc;
     using s_t = typename SType<TS, W>::type;
     constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
     constexpr uint32_t SB = sizeof(s_t), ROWB = W * sizeof(s_t);
     const int li = lane & 15, lg = lane >> 4;
-    const int rows = it.rows;
     const uint32_t slab = static_cast<uint32_t>(rows) * 64u;
-    const __amdgpu_buffer_rsrc_t Ar = make_rsrc(panel + it.data_off, slab * it.ktiles);
+    const __amdgpu_buffer_rsrc_t Ar = make_rsrc(pdata, slab * ktiles);
     const __amdgpu_buffer_rsrc_t Sr = make_rsrc(S, s_bytes);
     const __amdgpu_buffer_rsrc_t Cr =
-        make_rsrc(LEAF ? colidx + it.col_off : colidx, LEAF ? it.ktiles * TK * 4 : 0);
+        make_rsrc(LEAF ? colidx + col_off : colidx, LEAF ? ktiles * TK * 4 : 0);
     uint32_t aoff[NRT];
     bool aval[NRT];
 #pragma unroll
@@ -242,7 +248,7 @@ __device__ __forceinline__ void panel_body(const Item& it, const TS* panel, cons
 #pragma unroll
         for (int rt = 0; rt < NRT; rt++) {
             const uint32_t o = aval[rt] ? base + aoff[rt] : kOOB;
-            st.a[rt] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(Ar, o, 0, 0));
+            st.a[rt] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(Ar, o, 0, CP));
         }
 #pragma unroll
         for (int e = 0; e < EPL; e++) {
@@ -251,7 +257,7 @@ __device__ __forceinline__ void panel_body(const Item& it, const TS* panel, cons
                 o = static_cast<uint32_t>(ix.v[e]) * ROWB + bcol;
             } else {
                 const int k = kt * TK + lg * EPL + e;
-                o = k < it.ncols ? static_cast<uint32_t>(it.col_off + k) * ROWB + bcol : kOOB;
+                o = k < ncols ? static_cast<uint32_t>(col_off + k) * ROWB + bcol : kOOB;
             }
             st.b[e] = buf_load<s_t>(Sr, o);
         }
@@ -314,23 +320,23 @@ __device__ __forceinline__ void panel_body(const Item& it, const TS* panel, cons
     // column indices of the k-tile after it are in flight.  Loads past the end
     // of the panel (kt >= ktiles) fall outside the buffer and return zeros;
     // they are never consumed.
-    int kt = wave;
+    int kt = kt0;
     IdxV<EPL> ixA{}, ixB{};
     if constexpr (LEAF) {
         ixA = load_idx<EPL, TK>(Cr, kt, lg);
-        ixB = load_idx<EPL, TK>(Cr, kt + kWaves, lg);
+        ixB = load_idx<EPL, TK>(Cr, kt + kstep, lg);
     }
     Stage sA = load_stage(kt, ixA), sB;
-    while (kt < it.ktiles) {
-        if constexpr (LEAF) ixA = load_idx<EPL, TK>(Cr, kt + 2 * kWaves, lg);
-        sB = load_stage(kt + kWaves, ixB);
+    while (kt < ktiles) {
+        if constexpr (LEAF) ixA = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
+        sB = load_stage(kt + kstep, ixB);
         compute(sA);
-        kt += kWaves;
-        if (kt >= it.ktiles) break;
-        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 2 * kWaves, lg);
-        sA = load_stage(kt + kWaves, ixA);
+        kt += kstep;
+        if (kt >= ktiles) break;
+        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
+        sA = load_stage(kt + kstep, ixA);
         compute(sB);
-        kt += kWaves;
+        kt += kstep;
     }
 
     if constexpr (W == 16) {
@@ -341,7 +347,7 @@ __device__ __forceinline__ void panel_body(const Item& it, const TS* panel, cons
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = std::is_same<TS, double>::value ? 4 * v + lg : 4 * lg + v;
-                red[16 * rt + row][li] = acc[rt][v];
+                emit(16 * rt + row, li, static_cast<acc_t>(acc[rt][v]));
             }
     } else {
 #pragma unroll
@@ -349,57 +355,94 @@ __device__ __forceinline__ void panel_body(const Item& it, const TS* panel, cons
             acc_t v = acc[rt];
             v += __shfl_xor(v, 16);
             v += __shfl_xor(v, 32);
-            if (lg == 0) red[16 * rt + li][0] = v;
+            if (lg == 0) emit(16 * rt + li, 0, v);
         }
     }
 }
 
-// LEAF: leaf panel (indexed S rows, output scattered to Y through target_map);
-// otherwise far-field panel (contiguous S rows, output to partial rows or T).
-template <typename TS, int W, bool LEAF, typename TIO>
-__global__ __launch_bounds__(kWaves * 64) void k_panel(
+template <typename F>
+__device__ __forceinline__ void dispatch_nrt(int rows, F&& f) {
+    switch ((rows + 15) >> 4) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    default: f(std::integral_constant<int, 4>{}); break;
+    }
+}
+
+// Leaf panel: the kWaves waves split the k-tiles of one item; partial sums are
+// reduced through LDS and scattered to Y through target_map.
+template <typename TS, int W, int CP, typename TIO>
+__global__ __launch_bounds__(kWaves * 64) void k_leaf(
     const Item* __restrict__ items, const TS* __restrict__ panel,
-    const int32_t* __restrict__ colidx, typename SType<TS, W>::type* __restrict__ S, int64_t nsrc,
-    uint32_t s_bytes, typename Store<TS>::acc* __restrict__ partial,
-    const int32_t* __restrict__ tmap0, TIO* __restrict__ Y, int64_t ldy, int32_t rhs0,
-    int32_t nrhs) {
+    const int32_t* __restrict__ colidx, const typename SType<TS, W>::type* __restrict__ S,
+    uint32_t s_bytes, const int32_t* __restrict__ tmap0, TIO* __restrict__ Y, int64_t ldy,
+    int32_t rhs0, int32_t nrhs) {
     using acc_t = typename Store<TS>::acc;
-    using s_t = typename SType<TS, W>::type;
     const Item it = items[blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int rows = it.rows;
     __shared__ acc_t red[kWaves][64][W];
-    switch ((rows + 15) >> 4) {
-    case 1:
-        panel_body<TS, W, LEAF, 1>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
-        break;
-    case 2:
-        panel_body<TS, W, LEAF, 2>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
-        break;
-    case 3:
-        panel_body<TS, W, LEAF, 3>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
-        break;
-    default:
-        panel_body<TS, W, LEAF, 4>(it, panel, colidx, S, s_bytes, wave, lane, red[wave]);
-        break;
-    }
+    auto emit = [&](int row, int j, acc_t v) { red[wave][row][j] = v; };
+    dispatch_nrt(rows, [&](auto nrt) {
+        panel_body<TS, W, true, decltype(nrt)::value, CP>(panel + it.data_off, rows, it.ktiles,
+                                                          it.col_off, it.ncols, colidx, S, s_bytes,
+                                                          wave, kWaves, lane, emit);
+    });
     __syncthreads();
     for (int t = threadIdx.x; t < rows * W; t += kWaves * 64) {
         const int r = t / W, j = t % W;
         acc_t s = red[0][r][j];
 #pragma unroll
         for (int w = 1; w < kWaves; w++) s += red[w][r][j];
-        if constexpr (LEAF) {
-            if (rhs0 + j < nrhs)
-                Y[static_cast<int64_t>(tmap0[it.out_off + r]) + static_cast<int64_t>(rhs0 + j) * ldy] =
-                    static_cast<TIO>(s);
-        } else {
-            if (it.to_T)
-                S[(nsrc + it.out_off + r) * W + j] = from_double<s_t>(static_cast<double>(s));
-            else
-                partial[(it.out_off + r) * W + j] = s;
-        }
+        if (rhs0 + j < nrhs)
+            Y[static_cast<int64_t>(tmap0[it.out_off + r]) + static_cast<int64_t>(rhs0 + j) * ldy] =
+                static_cast<TIO>(s);
     }
+}
+
+// Far-field work item: up to kWaves consecutive 64-row groups of the stacked V
+// rows of one source cluster over one column chunk.  Wave w owns row group w
+// and walks every k-tile of the chunk; the waves read the same S rows (the
+// cluster's X rows), so the B operand is fetched once per workgroup and shared
+// through the L1 cache.  No cross-wave reduction is needed.
+struct FarItem {
+    int64_t data_off;  // panel of row group 0; group g at + g*64*ktiles*TK
+    int64_t col_off;   // first S row of the chunk
+    int64_t out_off;   // output row of row 0 of group 0 (group g: + 64*g)
+    int32_t ktiles;
+    int32_t ncols;
+    int32_t rows_last;  // rows of the last group (the others have 64)
+    int16_t nrg;        // number of row groups (<= kWaves)
+    int16_t to_T;       // 1: write T rows of S, 0: write partial rows
+};
+
+template <typename TS, int W, int CP>
+__global__ __launch_bounds__(kWaves * 64) void k_far(
+    const FarItem* __restrict__ items, const TS* __restrict__ panel,
+    typename SType<TS, W>::type* __restrict__ S, int64_t nsrc, uint32_t s_bytes,
+    typename Store<TS>::acc* __restrict__ partial) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    constexpr int TK = Store<TS>::TK;
+    const FarItem it = items[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave >= it.nrg) return;
+    const int rows = wave == it.nrg - 1 ? it.rows_last : 64;
+    const TS* pd = panel + it.data_off + static_cast<int64_t>(wave) * 64 * it.ktiles * TK;
+    const int64_t orow = it.out_off + 64 * wave;
+    auto emit = [&](int row, int j, acc_t v) {
+        if (row >= rows) return;
+        if (it.to_T)
+            S[(nsrc + orow + row) * W + j] = from_double<s_t>(static_cast<double>(v));
+        else
+            partial[(orow + row) * W + j] = v;
+    };
+    dispatch_nrt(rows, [&](auto nrt) {
+        panel_body<TS, W, false, decltype(nrt)::value, CP>(pd, rows, it.ktiles, it.col_off,
+                                                           it.ncols, nullptr, S, s_bytes, 0, 1,
+                                                           lane, emit);
+    });
 }
 
 // Sum the column-chunk partials of far-field row groups in a fixed order.
@@ -559,7 +602,7 @@ struct DeviceOp {
     int64_t nB = 0;
     int32_t* colidx = nullptr;
     int64_t ncol = 0;
-    Item* itemsA = nullptr;
+    FarItem* itemsA = nullptr;
     int64_t nA = 0;
     ReduceGroup* groups = nullptr;
     int64_t nG = 0;
@@ -597,20 +640,19 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
-// Work items are generated in cluster-tree order, i.e. spatially coherent:
-// neighbouring leaves share most near-field source rows and the T rows of the
-// low-rank blocks above them.  Workgroups are handed to the XCDs round-robin
-// (workgroup i -> XCD i % kXcds), each XCD with its own L2, so the item list is
-// cut into kXcds contiguous chunks of equal work and interleaved: every XCD
-// then walks one compact region of space in tree order.  HMX_ITEM_ORDER=lpt
-// selects largest-first order instead (better tail balance, no locality).
+// Dispatch order of the leaf items.  Default: largest first (longest
+// processing time first), the best measured order at N = 262,144.
+// HMX_ITEM_ORDER=xcd: the tree-ordered (spatially coherent) item list is cut
+// into kXcds contiguous chunks of equal work and interleaved, so that with
+// round-robin workgroup placement (workgroup i -> XCD i % kXcds) every XCD's L2
+// serves one compact region; HMX_ITEM_ORDER=tree keeps tree order.
 constexpr int kXcds = 8;
 
-void order_items_for_xcds(std::vector<Item>& items) {
+void order_leaf_items(std::vector<Item>& items) {
     if (items.size() < 2 * kXcds) return;
     auto work = [](const Item& it) { return int64_t(it.ktiles) * it.rows * 64 + 4096; };
     const char* env = std::getenv("HMX_ITEM_ORDER");
-    if (env && std::string(env) == "lpt") {
+    if (!env || std::string(env) == "lpt") {
         std::stable_sort(items.begin(), items.end(),
                          [&](const Item& a, const Item& b) { return work(a) > work(b); });
         return;
@@ -641,6 +683,14 @@ void order_items_for_xcds(std::vector<Item>& items) {
     items.swap(out);
 }
 
+// Far-field items: largest first (their B rows are shared inside the
+// workgroup, so dispatch order only matters for the tail).
+void order_far_items(std::vector<FarItem>& items) {
+    std::stable_sort(items.begin(), items.end(), [](const FarItem& a, const FarItem& b) {
+        return int64_t(a.ktiles) * a.nrg > int64_t(b.ktiles) * b.nrg;
+    });
+}
+
 }  // namespace
 
 void device_destroy(DeviceOp* op) {
@@ -665,7 +715,8 @@ int64_t device_panel_bytes(const DeviceOp* op) {
 }
 int64_t device_index_bytes(const DeviceOp* op) {
     if (!op) return 0;
-    return op->ncol * 4 + (op->nA + op->nB) * static_cast<int64_t>(sizeof(Item)) +
+    return op->ncol * 4 + op->nA * static_cast<int64_t>(sizeof(FarItem)) +
+           op->nB * static_cast<int64_t>(sizeof(Item)) +
            op->nG * static_cast<int64_t>(sizeof(ReduceGroup)) + (op->m + op->n) * 4;
 }
 
@@ -690,8 +741,12 @@ void device_build(HMatrix& h, int storage) {
     const int64_t CH = 2048;  // far-field column chunk (multiple of every TK)
 
     // -- far field: group low-rank blocks by source cluster -----------------------
-    std::vector<Item> itemsA;
+    // Per source cluster the rank rows of its blocks are stacked (T rows, in
+    // this order) and cut into 64-row groups; per column chunk the row-group
+    // panels are stored back to back and dispatched kWaves groups at a time.
+    std::vector<Item> itemsA;     // one per (row group, chunk): generation only
     std::vector<GenA> gensA;
+    std::vector<FarItem> itemsF;  // dispatch
     std::vector<ReduceGroup> groups;
     std::vector<int64_t> vrow;  // per T row: offset of its V row in the V payload
     std::vector<int64_t> trow_of_block(h.lowrank.size());
@@ -712,29 +767,41 @@ void device_build(HMatrix& h, int storage) {
                 for (int64_t j = 0; j < L.k; j++) vrow.push_back(L.voff + j * ncols);
                 t += L.k;
             }
+            const int64_t Rs = t - tstart;
+            const int64_t nrg = (Rs + 63) / 64;
             const int64_t nch = (ncols + CH - 1) / CH;
-            for (int64_t r0 = tstart; r0 < t; r0 += 64) {
-                const int32_t rows = static_cast<int32_t>(std::min<int64_t>(64, t - r0));
-                int64_t pb = -1;
-                if (nch > 1) {
-                    pb = npartial;
-                    npartial += rows * nch;
-                    groups.push_back({pb, r0, rows, static_cast<int32_t>(nch)});
-                }
-                for (int64_t c = 0; c < nch; c++) {
-                    const int64_t nc = std::min<int64_t>(CH, ncols - c * CH);
-                    const int64_t kt = (nc + TK - 1) / TK;
-                    Item it;
-                    it.data_off = data_off;
-                    it.col_off = (f.c0 - 1) + c * CH;
-                    it.out_off = nch > 1 ? pb + c * rows : r0;
-                    it.rows = rows;
-                    it.ktiles = static_cast<int32_t>(kt);
-                    it.ncols = static_cast<int32_t>(nc);
-                    it.to_T = nch > 1 ? 0 : 1;
-                    itemsA.push_back(it);
-                    gensA.push_back({r0, c * CH});
-                    data_off += kt * TK * rows;
+            int64_t pb = -1;
+            if (nch > 1) {
+                pb = npartial;
+                npartial += Rs * nch;
+                groups.push_back({pb, tstart, static_cast<int32_t>(Rs), static_cast<int32_t>(nch)});
+            }
+            for (int64_t c = 0; c < nch; c++) {
+                const int64_t nc = std::min<int64_t>(CH, ncols - c * CH);
+                const int64_t kt = (nc + TK - 1) / TK;
+                for (int64_t g0 = 0; g0 < nrg; g0 += kWaves) {
+                    const int64_t g1 = std::min<int64_t>(nrg, g0 + kWaves);
+                    FarItem fi;
+                    fi.data_off = data_off;
+                    fi.col_off = (f.c0 - 1) + c * CH;
+                    fi.out_off = nch > 1 ? pb + c * Rs + 64 * g0 : tstart + 64 * g0;
+                    fi.ktiles = static_cast<int32_t>(kt);
+                    fi.ncols = static_cast<int32_t>(nc);
+                    fi.nrg = static_cast<int16_t>(g1 - g0);
+                    fi.to_T = nch > 1 ? 0 : 1;
+                    for (int64_t g = g0; g < g1; g++) {
+                        const int32_t rows = static_cast<int32_t>(std::min<int64_t>(64, Rs - 64 * g));
+                        Item it{};
+                        it.data_off = data_off;
+                        it.rows = rows;
+                        it.ktiles = static_cast<int32_t>(kt);
+                        it.ncols = static_cast<int32_t>(nc);
+                        itemsA.push_back(it);
+                        gensA.push_back({tstart + 64 * g, c * CH});
+                        data_off += kt * TK * rows;
+                        fi.rows_last = rows;
+                    }
+                    itemsF.push_back(fi);
                 }
             }
         }
@@ -861,10 +928,10 @@ void device_build(HMatrix& h, int storage) {
                     (void*)dgen, (void*)dvrow, (void*)dA, (void*)dB})
         if (p) (void)hipFree(p);
 
-    order_items_for_xcds(itemsA);
-    order_items_for_xcds(itemsB);
-    op->itemsA = upload(itemsA);
-    op->nA = static_cast<int64_t>(itemsA.size());
+    order_far_items(itemsF);
+    order_leaf_items(itemsB);
+    op->itemsA = upload(itemsF);
+    op->nA = static_cast<int64_t>(itemsF.size());
     op->itemsB = upload(itemsB);
     op->nB = static_cast<int64_t>(itemsB.size());
     op->groups = upload(groups);
@@ -924,6 +991,8 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
     if (s_total >= (int64_t(1) << 32))
         throw Error(HMX_ERR_UNSUPPORTED, "source/T staging array exceeds 4 GiB (32-bit buffer offsets)");
     const uint32_t s_bytes = static_cast<uint32_t>(s_total);
+    // streamed panel loads marked non-temporal (HMX_PANEL_NT=1) or default policy
+
     for (int rhs0 = 0; rhs0 < nrhs; rhs0 += W) {
         {
             StageTimer t(op, st, 0);
@@ -932,9 +1001,8 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
         }
         if (op.nA) {
             StageTimer t(op, st, 1);
-            hipLaunchKernelGGL((k_panel<TS, W, false, TIO>), dim3(op.nA), dim3(kWaves * 64), 0, st,
-                               op.itemsA, panel, nullptr, S, op.n, s_bytes, P, nullptr, nullptr, 0,
-                               0, 0);
+            hipLaunchKernelGGL((k_far<TS, W, kPanelPolicy>), dim3(op.nA), dim3(kWaves * 64), 0, st,
+                               op.itemsA, panel, S, op.n, s_bytes, P);
         }
         if (op.nG) {
             StageTimer t(op, st, 2);
@@ -943,8 +1011,8 @@ void run(DeviceOp& op, const TIO* X, int64_t ldx, TIO* Y, int64_t ldy, int nrhs,
         }
         if (op.nB) {
             StageTimer t(op, st, 3);
-            hipLaunchKernelGGL((k_panel<TS, W, true, TIO>), dim3(op.nB), dim3(kWaves * 64), 0, st,
-                               op.itemsB, panel, op.colidx, S, op.n, s_bytes, P, op.tmap0, Y, ldy,
+            hipLaunchKernelGGL((k_leaf<TS, W, kPanelPolicy, TIO>), dim3(op.nB), dim3(kWaves * 64),
+                               0, st, op.itemsB, panel, op.colidx, S, s_bytes, op.tmap0, Y, ldy,
                                rhs0, nrhs);
         }
     }

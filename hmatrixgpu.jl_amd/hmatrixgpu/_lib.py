@@ -71,6 +71,15 @@ class HMXError(RuntimeError):
 
 
 def _load():
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  If
+    # libhmx.so were loaded first it would pull in the system copy and torch
+    # would later load its bundled copy next to it; the second runtime then
+    # finds the GPU already owned.  Importing torch first makes libhmx.so bind
+    # to the runtime torch already loaded (same SONAME).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libhmx.so not found at %s; build it with `make -C hmatrixgpu.jl_amd` "

@@ -238,10 +238,12 @@ class HMatrix:
         L.check(L.lib.hmx_profile_read(self._h, L.ptr_f64(ms), L.ptr_i64(n), int(bool(reset))))
         return ms, n
 
+    _destroy = L.lib.hmx_hmatrix_destroy  # kept on the class: safe at interpreter exit
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            L.lib.hmx_hmatrix_destroy(h)
+            self._destroy(h)
             self._h = None
 
 

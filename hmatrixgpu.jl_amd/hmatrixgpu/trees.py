@@ -95,10 +95,12 @@ class ClusterTree:
     def root(self):
         return self.node(0)
 
+    _destroy = L.lib.hmx_tree_destroy  # kept on the class: safe at interpreter exit
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            L.lib.hmx_tree_destroy(h)
+            self._destroy(h)
             self._h = None
 
 
@@ -169,10 +171,12 @@ class BlockTree:
         L.check(L.lib.hmx_blocktree_traverse(self._h, L.ptr_i64(d), L.ptr_i64(a)))
         return d[:self._nd], a[:self._na]
 
+    _destroy = L.lib.hmx_blocktree_destroy
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            L.lib.hmx_blocktree_destroy(h)
+            self._destroy(h)
             self._h = None
 
 
@@ -213,9 +217,11 @@ class _LowRank:
                                          V.ctypes.data_as(L.f64p)))
         return U, V
 
+    _destroy = L.lib.hmx_lowrank_destroy
+
     def __del__(self):
         if self._h is not None and self._h.value:
-            L.lib.hmx_lowrank_destroy(self._h)
+            self._destroy(self._h)
 
 
 def ACA_plus(n_rows, n_cols, calc_rows, calc_cols, eps, max_iter=None):
