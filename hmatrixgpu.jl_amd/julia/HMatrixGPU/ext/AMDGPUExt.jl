@@ -1,0 +1,24 @@
+# mul!/* on ROCArray device arrays -> hmx_matvec on the array's stream.
+module AMDGPUExt
+
+using HMatrixGPU
+using AMDGPU
+import LinearAlgebra: mul!
+import Base: *
+
+_io(::Type{Float64}) = HMatrixGPU.HMX_F64
+_io(::Type{Float32}) = HMatrixGPU.HMX_F32
+
+function mul!(result::ROCArray{T}, H::HMatrixGPU.HMatrix, x::ROCArray{T}) where {T<:Union{Float32,Float64}}
+    nrhs = ndims(x) == 1 ? 1 : size(x, 2)
+    HMatrixGPU.hmx_mul!(pointer(result), H, pointer(x), nrhs; ldx=size(x, 1), ldy=size(result, 1),
+                        io=_io(T), stream=AMDGPU.stream().stream)
+    return result
+end
+
+function *(H::HMatrixGPU.HMatrix, x::ROCArray{T}) where {T<:Union{Float32,Float64}}
+    result = ndims(x) == 1 ? AMDGPU.zeros(T, getfield(H, :m)) : AMDGPU.zeros(T, getfield(H, :m), size(x, 2))
+    return mul!(result, H, x)
+end
+
+end # module

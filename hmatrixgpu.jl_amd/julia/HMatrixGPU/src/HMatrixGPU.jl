@@ -1,0 +1,371 @@
+# HMatrixGPU -- Julia front end of the MI355X H-matrix engine.
+#
+# A thin ccall layer over libhmx.so (include/hmx.h) that keeps the public API of
+# HMatrixGPU.jl v0.1.0 (ClusterTree, BlockTree, ACA_plus, HMatrix, mul!, *, info,
+# set_backend, set_groupsize, @using_gpu) and the field names callers read.
+# Trees, block trees and the ACA+/SVD compression run natively in C++; the
+# product runs in gfx950 HIP kernels on device arrays (see ext/AMDGPUExt.jl).
+module HMatrixGPU
+
+using LinearAlgebra
+import LinearAlgebra: mul!
+import Base: *
+
+export ClusterTree, BlockTree, ACA_plus, HMatrix, info, set_groupsize, set_backend, @using_gpu
+
+const libhmx = get(ENV, "HMX_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib", "libhmx.so"))
+
+const HMX_F64, HMX_F32, HMX_BF16 = Int32(0), Int32(1), Int32(2)
+const HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D = Int32(0), Int32(1), Int32(2)
+
+function check(st)
+    st == 0 && return nothing
+    msg = unsafe_string(ccall((:hmx_last_error, libhmx), Cstring, ()))
+    error("libhmx error $st: $msg")
+end
+
+# ---- cluster tree (tree.jl:6-67) -----------------------------------------------
+mutable struct ClusterNode
+    is_leaf::Bool
+    start_idx::Int
+    end_idx::Int
+    center::Vector{Float64}
+    radius::Float64
+    left::Union{Nothing,ClusterNode}
+    right::Union{Nothing,ClusterNode}
+    id::Int
+end
+
+mutable struct ClusterTree
+    index_map::Vector{Int}
+    root::ClusterNode
+    nodes::Vector{ClusterNode}
+    handle::Ptr{Cvoid}
+end
+
+function ClusterTree(coordinates::Matrix{Float64}; max_points_per_leaf::Int=32, dims::Int=1)
+    dims <= 0 && error("Parameter `dims` must be a positive integer.")
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_tree_build, libhmx), Cint,
+                (Ptr{Float64}, Int64, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                coordinates, size(coordinates, 2), max_points_per_leaf, dims, out))
+    ni, nn = Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_tree_sizes, libhmx), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}), out[], ni, nn))
+    imap = Vector{Int64}(undef, ni[])
+    s, e = Vector{Int64}(undef, nn[]), Vector{Int64}(undef, nn[])
+    c, r = Matrix{Float64}(undef, 3, nn[]), Vector{Float64}(undef, nn[])
+    l, rt, lf = Vector{Int64}(undef, nn[]), Vector{Int64}(undef, nn[]), Vector{Int8}(undef, nn[])
+    check(ccall((:hmx_tree_export, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Int64}, Ptr{Int64}, Ptr{Int8}), out[], imap, s, e, c, r, l, rt, lf))
+    nodes = [ClusterNode(lf[k] != 0, s[k], e[k], c[:, k], r[k], nothing, nothing, k - 1)
+             for k in 1:nn[]]
+    for k in 1:nn[]
+        l[k] >= 0 && (nodes[k].left = nodes[l[k] + 1]; nodes[k].right = nodes[rt[k] + 1])
+    end
+    t = ClusterTree(imap, nodes[1], nodes, out[])
+    finalizer(x -> ccall((:hmx_tree_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), t)
+    return t
+end
+
+function info(tree::ClusterTree)  # tree.jl:153-187
+    d, mx, mn = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_tree_info, libhmx), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}),
+                tree.handle, d, mx, mn))
+    return Dict(:depth => d[], :max_points => mx[], :min_points => mn[])
+end
+
+# ---- block tree (block.jl:2-155) ----------------------------------------------------
+mutable struct BlockTreeNode
+    target_tree::Union{Nothing,ClusterNode}
+    source_tree::Union{Nothing,ClusterNode}
+    is_admissible::Bool
+    is_dense::Bool
+    left::Union{Nothing,BlockTreeNode}
+    right::Union{Nothing,BlockTreeNode}
+end
+
+mutable struct BlockTree
+    target_index_map::AbstractArray{Int}
+    source_index_map::AbstractArray{Int}
+    root::BlockTreeNode
+    handle::Ptr{Cvoid}
+    X::ClusterTree
+    Y::ClusterTree
+end
+
+function _export_blocktree(h::Ptr{Cvoid}, X::ClusterTree, Y::ClusterTree)
+    nn, nd, na = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_blocktree_sizes, libhmx), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}),
+                h, nn, nd, na))
+    n = nn[]
+    t, s = Vector{Int64}(undef, n), Vector{Int64}(undef, n)
+    a, d = Vector{Int8}(undef, n), Vector{Int8}(undef, n)
+    l, r = Vector{Int64}(undef, n), Vector{Int64}(undef, n)
+    check(ccall((:hmx_blocktree_export, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int8}, Ptr{Int8}, Ptr{Int64}, Ptr{Int64}),
+                h, t, s, a, d, l, r))
+    nodes = [BlockTreeNode(X.nodes[t[k] + 1], Y.nodes[s[k] + 1], a[k] != 0, d[k] != 0, nothing, nothing)
+             for k in 1:n]
+    for k in 1:n
+        l[k] >= 0 && (nodes[k].left = nodes[l[k] + 1]; nodes[k].right = nodes[r[k] + 1])
+    end
+    return nodes[1]
+end
+
+function BlockTree(X::ClusterTree, Y::ClusterTree; eta=1.5, index_map_using_cpu=true)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_blocktree_build, libhmx), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Ptr{Cvoid}}),
+                X.handle, Y.handle, Float64(eta), out))
+    bt = BlockTree(X.index_map, Y.index_map, _export_blocktree(out[], X, Y), out[], X, Y)
+    finalizer(x -> ccall((:hmx_blocktree_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), bt)
+    return bt
+end
+
+function merge_dense_matrices!(bt::BlockTree)  # block.jl:98-112
+    check(ccall((:hmx_blocktree_merge_dense, libhmx), Cint, (Ptr{Cvoid},), bt.handle))
+    bt.root = _export_blocktree(bt.handle, bt.X, bt.Y)
+    return nothing
+end
+
+function traverse(bt::BlockTree)  # block.jl:127-155
+    nn, nd, na = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_blocktree_sizes, libhmx), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}),
+                bt.handle, nn, nd, na))
+    d, a = Matrix{Int64}(undef, 2, nd[]), Matrix{Int64}(undef, 2, na[])
+    check(ccall((:hmx_blocktree_traverse, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
+                bt.handle, d, a))
+    pair(c) = (bt.X.nodes[c[1] + 1], bt.Y.nodes[c[2] + 1])
+    return [pair(c) for c in eachcol(d)], [pair(c) for c in eachcol(a)]
+end
+
+# ---- ACA+ and recompression (aca.jl:36-143) ------------------------------------------
+function _lowrank_arrays(h::Ptr{Cvoid})
+    m, n, k = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_lowrank_sizes, libhmx), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}), h, m, n, k))
+    U, V = Matrix{Float64}(undef, m[], k[]), Matrix{Float64}(undef, k[], n[])
+    check(ccall((:hmx_lowrank_export, libhmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h, U, V))
+    ccall((:hmx_lowrank_destroy, libhmx), Cint, (Ptr{Cvoid},), h)
+    return U, V
+end
+
+mutable struct _ACACtx
+    calc_rows::Any
+    calc_cols::Any
+    n_rows::Int
+    n_cols::Int
+end
+
+function _fill!(f, idx::Int64, out::Ptr{Float64}, len::Int)::Cint
+    try
+        unsafe_wrap(Vector{Float64}, out, len) .= vec(f(Int(idx)))
+        return Cint(0)
+    catch
+        return Cint(1)
+    end
+end
+function _row_cb(ctx::Ptr{Cvoid}, i::Int64, out::Ptr{Float64})::Cint
+    c = unsafe_pointer_to_objref(ctx)::_ACACtx
+    return _fill!(c.calc_rows, i, out, c.n_cols)
+end
+function _col_cb(ctx::Ptr{Cvoid}, j::Int64, out::Ptr{Float64})::Cint
+    c = unsafe_pointer_to_objref(ctx)::_ACACtx
+    return _fill!(c.calc_cols, j, out, c.n_rows)
+end
+
+function ACA_plus(n_rows, n_cols, calc_rows, calc_cols, eps; max_iter=n_rows)
+    ctx = _ACACtx(calc_rows, calc_cols, n_rows, n_cols)
+    rowfn = @cfunction(_row_cb, Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}))
+    colfn = @cfunction(_col_cb, Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve ctx begin
+        check(ccall((:hmx_aca_plus, libhmx), Cint,
+                    (Int64, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Int64, Ptr{Ptr{Cvoid}}),
+                    n_rows, n_cols, rowfn, colfn, pointer_from_objref(ctx), Float64(eps), max_iter, out))
+    end
+    return _lowrank_arrays(out[])
+end
+
+function SVD_recompress(U::Matrix, V::Matrix, eps::Float64)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_lowrank_create, libhmx), Cint,
+                (Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
+                size(U, 1), size(V, 2), size(U, 2), Matrix{Float64}(U), Matrix{Float64}(V), h))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    st = ccall((:hmx_svd_recompress, libhmx), Cint, (Ptr{Cvoid}, Float64, Ptr{Ptr{Cvoid}}), h[], eps, out)
+    ccall((:hmx_lowrank_destroy, libhmx), Cint, (Ptr{Cvoid},), h[])
+    check(st)
+    return _lowrank_arrays(out[])
+end
+
+# ---- kernels ---------------------------------------------------------------------------
+"""1/|x - y| with 0 where the points coincide; evaluated natively on host and GPU."""
+struct LaplaceKernel <: AbstractMatrix{Float64}
+    X::Matrix{Float64}
+    Y::Matrix{Float64}
+end
+Base.size(K::LaplaceKernel) = (size(K.X, 2), size(K.Y, 2))
+function Base.getindex(K::LaplaceKernel, i::Int, j::Int)
+    dx, dy, dz = K.X[1, i] - K.Y[1, j], K.X[2, i] - K.Y[2, j], K.X[3, i] - K.Y[3, j]
+    d = sqrt((dx * dx + dy * dy) + dz * dz)
+    return d == 0 ? 0.0 : 1.0 / d
+end
+_kernel_id(::LaplaceKernel) = HMX_KERNEL_LAPLACE
+_kernel_id(::AbstractMatrix) = HMX_KERNEL_CALLBACK
+
+# Any AbstractMatrix: entries through the getindex contract K[rows, cols]
+function _block_cb(ctx::Ptr{Cvoid}, rows::Ptr{Int64}, nr::Int64, cols::Ptr{Int64}, nc::Int64,
+                   out::Ptr{Float64})::Cint
+    K = unsafe_pointer_to_objref(ctx)
+    try
+        r = unsafe_wrap(Vector{Int64}, rows, nr)
+        c = unsafe_wrap(Vector{Int64}, cols, nc)
+        unsafe_wrap(Matrix{Float64}, out, (nr, nc)) .= K[r, c]
+        return Cint(0)
+    catch
+        return Cint(1)
+    end
+end
+
+# ---- H-matrix (hmatrix.jl:18-57, 72-148) ---------------------------------------------
+mutable struct HMatrix{T<:AbstractFloat}
+    K::AbstractMatrix
+    handle::Ptr{Cvoid}
+    storage::Int32
+    m::Int
+    n::Int
+end
+
+struct HMatrixCPU
+    K::AbstractMatrix
+    target_index_map::Vector{Int}
+    source_index_map::Vector{Int}
+    dense_block_indices::Vector{Tuple{Int,Int,Int,Int}}
+    approx_block_indices::Vector{Tuple{Int,Int,Int,Int}}
+    dense_blocks::Vector{Matrix}
+    U_matrices::Vector{Matrix}
+    V_matrices::Vector{Matrix}
+end
+
+"""
+    HMatrix(K, X, Y; eta=1.5, eps=1e-5, flatten=true, index_map_using_cpu=true,
+            svd_recompress=true, storage=Float64, device=0)
+
+Builds the H-matrix natively; with `flatten=true` the product operator is
+materialised on the GPU in `storage` precision (Float64, Float32, or :bf16).
+"""
+function HMatrix(K::AbstractMatrix, X::ClusterTree, Y::ClusterTree; eta=1.5, eps=1e-5,
+                 flatten=true, index_map_using_cpu=true, svd_recompress=true,
+                 storage=Float64, device=0)
+    st = storage === Float64 ? HMX_F64 : storage === Float32 ? HMX_F32 : HMX_BF16
+    mask = flatten ? Int32(1) << st : Int32(0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    kid = _kernel_id(K)
+    cb = @cfunction(_block_cb, Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}))
+    GC.@preserve K begin
+        xt = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.X)
+        xs = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.Y)
+        check(ccall((:hmx_hmatrix_build, libhmx), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}, Any,
+                     Float64, Float64, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                    X.handle, Y.handle, kid, xt, xs, cb, K, Float64(eta), Float64(eps),
+                    Int32(svd_recompress), mask, Int32(device), Int32(0), out))
+    end
+    m, n = size(K)
+    T = storage === Float64 ? Float64 : Float32
+    H = HMatrix{T}(K, out[], st, m, n)
+    finalizer(x -> ccall((:hmx_hmatrix_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), H)
+    flatten && return H
+    return _cpu_view(H)
+end
+
+function _sizes(H::HMatrix)
+    s = Vector{Int64}(undef, 8)
+    check(ccall((:hmx_hmatrix_sizes, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}), H.handle, s))
+    return s
+end
+
+"""The reference's flattened fields (hmatrix.jl:46-57) exported from the library."""
+function Base.getproperty(H::HMatrix, f::Symbol)
+    if f in (:target_index_map, :source_index_map, :dense_block_indices, :U_block_indices,
+             :V_block_indices, :dense_blocks, :U_matrices, :V_matrices, :Vx_buffer)
+        return _layout(H)[f]
+    end
+    return getfield(H, f)
+end
+
+function _layout(H::HMatrix)
+    m, n, nd, na, R, dl, ul, vl = _sizes(H)
+    tm, sm = Vector{Int64}(undef, m), Vector{Int64}(undef, n)
+    di, ui, vi = Matrix{Int64}(undef, 5, nd), Matrix{Int64}(undef, 5, na), Matrix{Int64}(undef, 3, R)
+    D, U, V = Vector{Float64}(undef, dl), Vector{Float64}(undef, ul), Vector{Float64}(undef, vl)
+    check(ccall((:hmx_hmatrix_export_reference_layout, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), getfield(H, :handle), tm, sm, di, ui, vi, D, U, V))
+    return Dict(:target_index_map => tm, :source_index_map => sm, :dense_block_indices => di,
+                :U_block_indices => ui, :V_block_indices => vi, :dense_blocks => D,
+                :U_matrices => U, :V_matrices => V, :Vx_buffer => zeros(R))
+end
+
+function _cpu_view(H::HMatrix)
+    L = _layout(H)
+    m, n, nd, na, R = _sizes(H)[1:5]
+    dr, ar = Matrix{Int64}(undef, 4, nd), Matrix{Int64}(undef, 4, na)
+    check(ccall((:hmx_hmatrix_export_block_ranges, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}),
+                getfield(H, :handle), dr, ar))
+    di, ui, vi = L[:dense_block_indices], L[:U_block_indices], L[:V_block_indices]
+    Ds = Matrix[permutedims(reshape(L[:dense_blocks][di[5, b]+1:di[5, b]+(di[2, b]-di[1, b]+1)*(di[4, b]-di[3, b]+1)],
+                                    di[4, b] - di[3, b] + 1, di[2, b] - di[1, b] + 1)) for b in 1:nd]
+    Us, Vs = Matrix[], Matrix[]
+    for b in 1:na
+        mb, k, nb = ar[2, b] - ar[1, b] + 1, ui[4, b] - ui[3, b] + 1, ar[4, b] - ar[3, b] + 1
+        push!(Us, permutedims(reshape(L[:U_matrices][ui[5, b]+1:ui[5, b]+mb*k], k, mb)))
+        voff = k > 0 ? vi[3, ui[3, b]] : 0
+        push!(Vs, permutedims(reshape(L[:V_matrices][voff+1:voff+k*nb], nb, k)))
+    end
+    return HMatrixCPU(getfield(H, :K), L[:target_index_map], L[:source_index_map],
+                      [Tuple(c) for c in eachcol(dr)], [Tuple(c) for c in eachcol(ar)], Ds, Us, Vs)
+end
+
+function info(H::HMatrix)  # hmatrix.jl:289-330 plus byte counts
+    o = Vector{Float64}(undef, 16)
+    check(ccall((:hmx_hmatrix_info, libhmx), Cint, (Ptr{Cvoid}, Ptr{Float64}), getfield(H, :handle), o))
+    return Dict("data_type" => Float64, "size" => (getfield(H, :m), getfield(H, :n)),
+                "leaves" => Int(o[1]), "admissible_leaves" => Int(o[2]), "full_leaves" => Int(o[3]),
+                "min_rank" => Int(o[4]), "max_rank" => Int(o[5]), "min_dense_size" => Int(o[6]),
+                "max_dense_size" => Int(o[7]), "compression_ratio" => o[8])
+end
+
+"""
+    hmx_mul!(result_ptr, H, x_ptr, nrhs; ldx, ldy, io, stream)
+
+Raw device-pointer product (mult.jl:159-189 semantics, column by column).
+Device array types wire their `mul!` to this (ext/AMDGPUExt.jl).
+"""
+function hmx_mul!(result_ptr::Ptr, H::HMatrix, x_ptr::Ptr, nrhs::Integer;
+                  ldx=getfield(H, :n), ldy=getfield(H, :m), io::Int32=HMX_F64, stream::Ptr=C_NULL)
+    check(ccall((:hmx_matvec, libhmx), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int32, Int32, Ptr{Cvoid}),
+                getfield(H, :handle), getfield(H, :storage), x_ptr, ldx, result_ptr, ldy,
+                Int32(nrhs), io, stream))
+    return nothing
+end
+
+# ---- backend registry (HMatrixGPU.jl:14-109) -----------------------------------------
+const groupsize = Ref(512)
+set_groupsize(x) = (groupsize[] = x)  # accepted; the gfx950 kernels pick their own shape
+function set_backend(backend="amd")
+    ok = lowercase(backend) in ("amd", "roc", "amdgpu")
+    ok || @info("This build of HMatrixGPU runs on AMD MI355X (gfx950) only")
+    return ok
+end
+macro using_gpu()
+    quote
+        try
+            using AMDGPU
+        catch
+        end
+    end
+end
+
+end # module
