@@ -1,0 +1,441 @@
+// batch.hip -- products of many independent H-matrices in one graph launch.
+//
+// No reference counterpart beyond mul! itself (mult.jl:159-189): the reference
+// launches two kernels per H-matrix.  Here the work lists of all instances
+// (far-field items, partial-reduce groups, leaf items) are concatenated, each
+// entry tagged with its instance, and every stage is ONE kernel over all
+// instances; the kernels read the instance's arrays and scratch from a device
+// table of InstDesc.  The four stage launches are captured once into a hipGraph
+// (on a private stream) and replayed with a single hipGraphLaunch per call on
+// the caller's stream.  The per-call X / Y pointers live in the device table,
+// which is refreshed by an asynchronous copy from pinned host memory before the
+// graph launch (only when they changed).
+//
+// Tuning: the RHS path (VALU 4-wide vs MFMA 16-wide when nrhs <= 4) and the
+// leaf dispatch order (largest first / tree / XCD chunks) are chosen by timing
+// the candidates on the actual batch (hmx_batch_autotune); the choice can be
+// cached in a file keyed by the batch's shape.
+#include "device_op.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+namespace hmx {
+
+namespace {
+
+constexpr int kRing = 4;  // pinned staging slots for the per-call descriptor table
+
+struct BatchCfg {
+    int W = 16;
+    int order = 0;  // 0 largest first, 1 tree, 2 XCD chunks
+};
+
+const char* order_name(int o) { return o == 1 ? "tree" : o == 2 ? "xcd" : "lpt"; }
+
+}  // namespace
+
+struct Batch {
+    int device = 0, storage = 0, nrhs = 1, io = HMX_F32;
+    std::vector<HMatrix*> hs;
+    std::vector<DeviceOp*> ops;
+    BatchCfg cfg;
+    // per RHS width: per-instance scratch
+    std::vector<ScratchW> scratch[3];
+    // device work lists and descriptor table
+    FarItem* dA = nullptr;
+    ReduceGroup* dG = nullptr;
+    Item* dB = nullptr;
+    StageArrays arrays;
+    InstDesc* dInst = nullptr;
+    std::vector<InstDesc> hInst;  // host image of the table
+    InstDesc* pinned[kRing] = {};
+    hipEvent_t slot_ev[kRing] = {};
+    int slot = 0;
+    bool table_dirty = true;
+    // graph
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t payload_bytes = 0;
+    ~Batch();
+};
+
+namespace {
+
+void free_lists(Batch& b) {
+    for (void* p : {(void*)b.dA, (void*)b.dG, (void*)b.dB})
+        if (p) (void)hipFree(p);
+    b.dA = nullptr;
+    b.dG = nullptr;
+    b.dB = nullptr;
+}
+
+void drop_graph(Batch& b) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+}
+
+ScratchW& batch_scratch(Batch& b, size_t i, int W) {
+    auto& v = b.scratch[w_slot(W)];
+    if (v.empty()) v.resize(b.ops.size());
+    ScratchW& s = v[i];
+    if (!s.S) {
+        const DeviceOp& op = *b.ops[i];
+        const size_t sb = static_cast<size_t>(s_rows(op)) * W * s_elem(op.storage, W);
+        if (sb >= (size_t(1) << 32))
+            throw Error(HMX_ERR_UNSUPPORTED, "source/T staging array exceeds 4 GiB");
+        HIPCHECK(hipMalloc(&s.S, sb));
+        HIPCHECK(hipMemset(s.S, 0, sb));
+        HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
+    }
+    return s;
+}
+
+// (Re)build the concatenated work lists and the descriptor table for cfg.
+void build_lists(Batch& b) {
+    free_lists(b);
+    drop_graph(b);
+    std::vector<FarItem> A;
+    std::vector<ReduceGroup> G;
+    std::vector<Item> B;
+    int64_t gmax = 0;
+    b.hInst.assign(b.ops.size(), InstDesc{});
+    for (size_t i = 0; i < b.ops.size(); i++) {
+        const DeviceOp& op = *b.ops[i];
+        for (FarItem f : op.hItemsA) {
+            f.inst = static_cast<int32_t>(i);
+            A.push_back(f);
+        }
+        for (ReduceGroup g : op.hGroups) {
+            g.inst = static_cast<int32_t>(i);
+            G.push_back(g);
+        }
+        for (Item it : op.hItemsB) {
+            it.inst = static_cast<int32_t>(i);
+            B.push_back(it);
+        }
+        gmax = std::max(gmax, op.n);
+        ScratchW& sc = batch_scratch(b, i, b.cfg.W);
+        InstDesc& d = b.hInst[i];
+        d.panel = op.panel;
+        d.colidx = op.colidx;
+        d.S = sc.S;
+        d.partial = sc.P;
+        d.tmap0 = op.tmap0;
+        d.inv_smap = op.inv_smap;
+        d.n = op.n;
+        d.s_bytes = static_cast<uint32_t>(static_cast<size_t>(s_rows(op)) * b.cfg.W *
+                                          s_elem(op.storage, b.cfg.W));
+        d.nrhs = b.nrhs;
+        d.rhs0 = 0;
+    }
+    std::stable_sort(A.begin(), A.end(), [](const FarItem& x, const FarItem& y) {
+        return int64_t(x.ktiles) * x.nrg > int64_t(y.ktiles) * y.nrg;
+    });
+    order_leaf_items(B, order_name(b.cfg.order));
+    b.dA = upload(A);
+    b.dG = upload(G);
+    b.dB = upload(B);
+    b.arrays = StageArrays{};
+    b.arrays.itemsA = b.dA;
+    b.arrays.nA = static_cast<int64_t>(A.size());
+    b.arrays.groups = b.dG;
+    b.arrays.nG = static_cast<int64_t>(G.size());
+    b.arrays.itemsB = b.dB;
+    b.arrays.nB = static_cast<int64_t>(B.size());
+    b.arrays.gather_n = gmax;
+    b.arrays.gather_insts = static_cast<int>(b.ops.size());
+    b.table_dirty = true;
+}
+
+template <typename TS, int W, typename TIO>
+void capture_stages(Batch& b) {
+    launch_stages<TS, W, TIO, true>(b.arrays, InstDesc{}, b.dInst, b.cap_stream, nullptr);
+}
+
+template <typename TS, typename TIO>
+void capture_w(Batch& b) {
+    if (b.cfg.W == 1)
+        capture_stages<TS, 1, TIO>(b);
+    else if (b.cfg.W == 4)
+        capture_stages<TS, 4, TIO>(b);
+    else
+        capture_stages<TS, 16, TIO>(b);
+}
+
+void capture(Batch& b) {
+    drop_graph(b);
+    HIPCHECK(hipStreamBeginCapture(b.cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+        if (b.storage == HMX_F64)
+            b.io == HMX_F64 ? capture_w<double, double>(b) : capture_w<double, float>(b);
+        else if (b.storage == HMX_F32)
+            b.io == HMX_F64 ? capture_w<float, double>(b) : capture_w<float, float>(b);
+        else
+            b.io == HMX_F64 ? capture_w<bf16s, double>(b) : capture_w<bf16s, float>(b);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b.cap_stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(b.cap_stream, &b.graph));
+    HIPCHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+void set_io(Batch& b, const void* const* X, const int64_t* ldx, void* const* Y, const int64_t* ldy) {
+    for (size_t i = 0; i < b.hInst.size(); i++) {
+        InstDesc& d = b.hInst[i];
+        const int64_t lx = ldx ? ldx[i] : b.hs[i]->n;
+        const int64_t ly = ldy ? ldy[i] : b.hs[i]->m;
+        if (!X[i] || !Y[i]) throw Error(HMX_ERR_ARG, "null X or Y in batch");
+        if (lx < b.hs[i]->n || ly < b.hs[i]->m)
+            throw Error(HMX_ERR_ARG, "leading dimension smaller than the size");
+        if (d.X != X[i] || d.Y != Y[i] || d.ldx != lx || d.ldy != ly) {
+            d.X = X[i];
+            d.Y = Y[i];
+            d.ldx = lx;
+            d.ldy = ly;
+            b.table_dirty = true;
+        }
+    }
+}
+
+void launch(Batch& b, hipStream_t st) {
+    if (b.table_dirty) {
+        const size_t bytes = b.hInst.size() * sizeof(InstDesc);
+        HIPCHECK(hipEventSynchronize(b.slot_ev[b.slot]));  // the slot's last copy has run
+        std::memcpy(b.pinned[b.slot], b.hInst.data(), bytes);
+        HIPCHECK(hipMemcpyAsync(b.dInst, b.pinned[b.slot], bytes, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipEventRecord(b.slot_ev[b.slot], st));
+        b.slot = (b.slot + 1) % kRing;
+        b.table_dirty = false;
+    }
+    if (!b.exec) capture(b);
+    HIPCHECK(hipGraphLaunch(b.exec, st));
+}
+
+std::string shape_key(const Batch& b) {
+    std::ostringstream k;
+    k << "s" << b.storage << "_r" << b.nrhs << "_io" << b.io << "_c" << b.ops.size();
+    uint64_t h = 1469598103934665603ull;
+    for (const DeviceOp* op : b.ops) {
+        for (int64_t v : {op->m, op->n, op->panel_elems, op->nA, op->nB}) {
+            h ^= static_cast<uint64_t>(v);
+            h *= 1099511628211ull;
+        }
+    }
+    k << "_" << std::hex << h;
+    return k.str();
+}
+
+bool cache_lookup(const std::string& key, BatchCfg* cfg) {
+    const char* path = std::getenv("HMX_TUNE_CACHE");
+    if (!path) return false;
+    std::ifstream f(path);
+    std::string k;
+    int W, order;
+    while (f >> k >> W >> order)
+        if (k == key) {
+            cfg->W = W;
+            cfg->order = order;
+            return true;
+        }
+    return false;
+}
+
+void cache_store(const std::string& key, const BatchCfg& cfg) {
+    const char* path = std::getenv("HMX_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << key << " " << cfg.W << " " << cfg.order << "\n";
+}
+
+}  // namespace
+
+Batch::~Batch() {
+    (void)hipSetDevice(device);
+    for (auto& e : slot_ev)
+        if (e) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
+    drop_graph(*this);
+    free_lists(*this);
+    if (dInst) (void)hipFree(dInst);
+    for (auto& p : pinned)
+        if (p) (void)hipHostFree(p);
+    for (auto& v : scratch)
+        for (auto& s : v)
+            for (void* p : {s.S, s.P})
+                if (p) (void)hipFree(p);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+}
+
+}  // namespace hmx
+
+struct hmx_batch {
+    std::unique_ptr<hmx::Batch> b;
+};
+
+namespace {
+
+template <typename F>
+int bguard(F&& f) {
+    try {
+        f();
+        return HMX_OK;
+    } catch (const hmx::Error& e) {
+        hmx::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        hmx::set_last_error("out of host memory");
+        return HMX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        hmx::set_last_error(e.what());
+        return HMX_ERR_ARG;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int hmx_batch_create(hmx_hmatrix* const* hs, int32_t count, int32_t storage, int32_t nrhs,
+                     int32_t io_dtype, hmx_batch** out) {
+    return bguard([&] {
+        if (!hs || !out || count <= 0 || nrhs <= 0 || nrhs > 16)
+            throw hmx::Error(HMX_ERR_ARG, "hmx_batch_create: bad argument (1 <= nrhs <= 16)");
+        if (storage < 0 || storage > 2 || (io_dtype != HMX_F64 && io_dtype != HMX_F32))
+            throw hmx::Error(HMX_ERR_ARG, "hmx_batch_create: bad storage or io_dtype");
+        *out = nullptr;
+        auto b = std::make_unique<hmx::Batch>();
+        b->storage = storage;
+        b->nrhs = nrhs;
+        b->io = io_dtype;
+        for (int32_t i = 0; i < count; i++) {
+            if (!hs[i] || !hs[i]->h) throw hmx::Error(HMX_ERR_ARG, "hmx_batch_create: NULL H-matrix");
+            hmx::HMatrix* h = hs[i]->h.get();
+            if (!h->dev[storage])
+                throw hmx::Error(HMX_ERR_STATE, "batch member without the requested storage on device");
+            if (i > 0 && h->device != b->device)
+                throw hmx::Error(HMX_ERR_ARG, "batch members on different devices");
+            b->device = h->device;
+            b->hs.push_back(h);
+            b->ops.push_back(h->dev[storage].get());
+            b->payload_bytes += hmx::device_panel_bytes(h->dev[storage].get());
+        }
+        hmx::DeviceGuard guard(b->device);
+        b->cfg.W = nrhs == 1 ? 1 : nrhs <= 4 ? 4 : 16;
+        hmx::BatchCfg cached;
+        if (hmx::cache_lookup(hmx::shape_key(*b), &cached) &&
+            (cached.W == b->cfg.W || (nrhs <= 4 && cached.W == 16)))
+            b->cfg = cached;
+        HIPCHECK(hipMalloc(&b->dInst, count * sizeof(hmx::InstDesc)));
+        for (int k = 0; k < hmx::kRing; k++) {
+            HIPCHECK(hipHostMalloc(&b->pinned[k], count * sizeof(hmx::InstDesc), hipHostMallocDefault));
+            HIPCHECK(hipEventCreateWithFlags(&b->slot_ev[k], hipEventDisableTiming));
+        }
+        HIPCHECK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        hmx::build_lists(*b);
+        *out = new hmx_batch{std::move(b)};
+    });
+}
+
+int hmx_batch_matvec(hmx_batch* hb, const void* const* X, const int64_t* ldx, void* const* Y,
+                     const int64_t* ldy, void* stream) {
+    return bguard([&] {
+        if (!hb || !hb->b || !X || !Y) throw hmx::Error(HMX_ERR_ARG, "hmx_batch_matvec: NULL argument");
+        hmx::Batch& b = *hb->b;
+        hmx::DeviceGuard guard(b.device);
+        hmx::set_io(b, X, ldx, Y, ldy);
+        hmx::launch(b, static_cast<hipStream_t>(stream));
+    });
+}
+
+int hmx_batch_set_config(hmx_batch* hb, int32_t rhs_width, int32_t order) {
+    return bguard([&] {
+        if (!hb || !hb->b) throw hmx::Error(HMX_ERR_ARG, "hmx_batch_set_config: NULL");
+        hmx::Batch& b = *hb->b;
+        const bool ok_w = rhs_width == 16 || (rhs_width == 4 && b.nrhs <= 4 && b.nrhs > 1) ||
+                          (rhs_width == 1 && b.nrhs == 1);
+        if (!ok_w || order < 0 || order > 2)
+            throw hmx::Error(HMX_ERR_ARG, "hmx_batch_set_config: unsupported rhs_width / order");
+        hmx::DeviceGuard guard(b.device);
+        HIPCHECK(hipDeviceSynchronize());
+        b.cfg.W = rhs_width;
+        b.cfg.order = order;
+        hmx::build_lists(b);
+    });
+}
+
+int hmx_batch_get_config(const hmx_batch* hb, int32_t* rhs_width, int32_t* order) {
+    return bguard([&] {
+        if (!hb || !hb->b || !rhs_width || !order) throw hmx::Error(HMX_ERR_ARG, "NULL");
+        *rhs_width = hb->b->cfg.W;
+        *order = hb->b->cfg.order;
+    });
+}
+
+int hmx_batch_autotune(hmx_batch* hb, const void* const* X, const int64_t* ldx, void* const* Y,
+                       const int64_t* ldy, int32_t iters, void* stream, double* best_ms) {
+    return bguard([&] {
+        if (!hb || !hb->b || !X || !Y) throw hmx::Error(HMX_ERR_ARG, "hmx_batch_autotune: NULL");
+        hmx::Batch& b = *hb->b;
+        hmx::DeviceGuard guard(b.device);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        std::vector<hmx::BatchCfg> cands;
+        std::vector<int> widths;
+        if (b.nrhs == 1)
+            widths = {1};
+        else if (b.nrhs <= 4)
+            widths = {4, 16};
+        else
+            widths = {16};
+        for (int W : widths)
+            for (int o = 0; o < 3; o++) cands.push_back({W, o});
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        double best = 1e300;
+        hmx::BatchCfg best_cfg = b.cfg;
+        const int n_it = iters > 0 ? iters : 5;
+        for (const auto& c : cands) {
+            HIPCHECK(hipStreamSynchronize(st));
+            b.cfg = c;
+            hmx::build_lists(b);
+            hmx::set_io(b, X, ldx, Y, ldy);
+            hmx::launch(b, st);  // capture + warm-up
+            hmx::launch(b, st);
+            HIPCHECK(hipEventRecord(e0, st));
+            for (int k = 0; k < n_it; k++) hmx::launch(b, st);
+            HIPCHECK(hipEventRecord(e1, st));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / n_it;
+            if (per < best) {
+                best = per;
+                best_cfg = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b.cfg = best_cfg;
+        hmx::build_lists(b);
+        hmx::set_io(b, X, ldx, Y, ldy);
+        hmx::cache_store(hmx::shape_key(b), best_cfg);
+        if (best_ms) *best_ms = best;
+    });
+}
+
+int hmx_batch_destroy(hmx_batch* hb) {
+    delete hb;
+    return HMX_OK;
+}
+
+}  // extern "C"

@@ -19,13 +19,16 @@ struct hmx_blocktree {
 struct hmx_lowrank {
     hmx::LowRank lr;
 };
-struct hmx_hmatrix {
-    std::unique_ptr<hmx::HMatrix> h;
-};
 
 namespace {
 
 thread_local std::string g_err;
+
+}  // namespace
+
+void hmx::set_last_error(const char* msg) { g_err = msg; }
+
+namespace {
 
 template <typename F>
 int guarded(F&& f) {

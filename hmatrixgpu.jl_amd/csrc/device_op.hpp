@@ -1,0 +1,138 @@
+// device_op.hpp -- host-side state of a device-resident H-matrix operator and
+// the stage launcher shared by the single and batched products.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace hmx {
+
+struct ScratchW {
+    void* S = nullptr;  // (n + R + 1 + TK) x W source rows
+    void* P = nullptr;  // far-field partial rows x W
+};
+
+struct Scratch {
+    ScratchW w[3];  // W = 1, 4, 16
+};
+
+inline int w_slot(int W) { return W == 1 ? 0 : W == 4 ? 1 : 2; }
+
+struct DeviceOp {
+    int storage = 0, device = 0;
+    int64_t m = 0, n = 0, R = 0;
+    void* panel = nullptr;
+    int64_t panel_elems = 0;
+    Item* itemsB = nullptr;
+    int64_t nB = 0;
+    int32_t* colidx = nullptr;
+    int64_t ncol = 0;
+    FarItem* itemsA = nullptr;
+    int64_t nA = 0;
+    ReduceGroup* groups = nullptr;
+    int64_t nG = 0;
+    int64_t npartial = 0;
+    int32_t* tmap0 = nullptr;
+    int32_t* inv_smap = nullptr;
+    // host copies of the work lists (the batched operator re-packs them)
+    std::vector<Item> hItemsB;
+    std::vector<FarItem> hItemsA;
+    std::vector<ReduceGroup> hGroups;
+    std::mutex mu;
+    std::map<void*, Scratch> scratch;
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+};
+
+inline size_t elem_size(int storage) { return storage == HMX_F64 ? 8 : storage == HMX_F32 ? 4 : 2; }
+inline int tk_of(int storage) { return static_cast<int>(64 / elem_size(storage)); }
+inline int64_t s_rows(const DeviceOp& op) { return op.n + op.R + 1 + tk_of(op.storage); }
+// element size of S / partials for a storage precision and RHS width
+inline size_t s_elem(int storage, int W) {
+    return storage == HMX_F64 ? 8 : (W == 16 && storage == HMX_BF16 ? 2 : 4);
+}
+inline size_t p_elem(int storage) { return storage == HMX_F64 ? 8 : 4; }
+
+template <typename T>
+T* upload(const std::vector<T>& v) {
+    T* d = nullptr;
+    if (v.empty()) return nullptr;
+    HIPCHECK(hipMalloc(&d, v.size() * sizeof(T)));
+    HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        HIPCHECK(hipGetDevice(&prev));
+        if (prev != dev) HIPCHECK(hipSetDevice(dev));
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+// Work lists of one launch (one H-matrix, or a batch of them).
+struct StageArrays {
+    const FarItem* itemsA = nullptr;
+    int64_t nA = 0;
+    const ReduceGroup* groups = nullptr;
+    int64_t nG = 0;
+    const Item* itemsB = nullptr;
+    int64_t nB = 0;
+    int64_t gather_n = 0;   // largest source size over the instances
+    int gather_insts = 1;   // instances (grid.y of the gather)
+};
+
+// HIP-event timing of one stage on the launch stream (only when profiling).
+struct StageTimer {
+    DeviceOp* op;
+    hipStream_t st;
+    int stage;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(DeviceOp* o, hipStream_t s, int k) : op(o), st(s), stage(k) {
+        if (!op || !op->profiling) return;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        HIPCHECK(hipEventRecord(a, st));
+    }
+    ~StageTimer() {
+        if (!op || !op->profiling || !a) return;
+        (void)hipEventRecord(b, st);
+        std::lock_guard<std::mutex> g(op->mu);
+        op->ev[stage].push_back({a, b});
+    }
+};
+
+// The four stages of a product pass: gather X -> S, far-field panels (V*X),
+// partial reduce, leaf panels -> Y.
+template <typename TS, int W, typename TIO, bool BATCH>
+void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* insts,
+                   hipStream_t st, DeviceOp* prof) {
+    {
+        StageTimer t(prof, st, 0);
+        hipLaunchKernelGGL((k_gather<TS, W, TIO, BATCH>),
+                           dim3(static_cast<unsigned>((A.gather_n + 255) / 256), A.gather_insts),
+                           dim3(256), 0, st, one, insts);
+    }
+    if (A.nA) {
+        StageTimer t(prof, st, 1);
+        hipLaunchKernelGGL((k_far<TS, W, kPanelPolicy, BATCH>), dim3(A.nA), dim3(kWaves * 64), 0,
+                           st, A.itemsA, one, insts);
+    }
+    if (A.nG) {
+        StageTimer t(prof, st, 2);
+        hipLaunchKernelGGL((k_reduce<TS, W, BATCH>), dim3(A.nG), dim3(256), 0, st, A.groups, one,
+                           insts);
+    }
+    if (A.nB) {
+        StageTimer t(prof, st, 3);
+        hipLaunchKernelGGL((k_leaf<TS, W, kPanelPolicy, TIO, BATCH>), dim3(A.nB),
+                           dim3(kWaves * 64), 0, st, A.itemsB, one, insts);
+    }
+}
+
+// Dispatch order of leaf work items (see device.hip).
+void order_leaf_items(std::vector<Item>& items, const char* order);
+
+}  // namespace hmx

@@ -152,4 +152,12 @@ int64_t device_index_bytes(const DeviceOp* op);
 void device_profile_enable(HMatrix& h, bool on);
 void device_profile_read(HMatrix& h, double* ms, int64_t* launches, bool reset);
 
+// Thread-local message returned by hmx_last_error().
+void set_last_error(const char* msg);
+
 }  // namespace hmx
+
+// Opaque ABI handle of an H-matrix (include/hmx.h).
+struct hmx_hmatrix {
+    std::unique_ptr<hmx::HMatrix> h;
+};

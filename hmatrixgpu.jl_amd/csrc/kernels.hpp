@@ -1,0 +1,642 @@
+// kernels.hpp -- gfx950 device types and kernels of the H-matrix product and
+// of the panel generation.  Included by device.hip (one H-matrix per launch) and
+// batch.hip (many H-matrices per launch).  The layout and the stages are
+// described in device.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hmx_host.hpp"
+
+namespace hmx {
+
+#define HIPCHECK(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            throw ::hmx::Error(HMX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct bf16s {
+    uint16_t u;
+};
+
+// One panel work item (rows <= 64).
+struct Item {
+    int64_t data_off;  // element offset of the panel in the storage array
+    int64_t col_off;   // leaf panels: offset into colidx; far-field: first S row
+    int64_t out_off;   // leaf panels: first permuted target row; far-field: output row
+    int32_t rows;
+    int32_t ktiles;
+    int32_t ncols;     // far-field generation: valid columns of the chunk
+    int32_t inst;      // instance (H-matrix) index in a batched launch
+};
+
+struct ReduceGroup {
+    int64_t pbase;  // first partial row
+    int64_t tbase;  // first T row (S row = n + tbase)
+    int32_t rows;
+    int32_t nch;
+    int32_t inst;
+    int32_t pad;
+};
+
+// Per-instance device state of one product launch: the H-matrix operator's
+// arrays, its per-stream scratch and the call's X / Y.  Passed by value for a
+// single H-matrix, read from a device table for a batch of them.
+struct InstDesc {
+    const void* panel;
+    const int32_t* colidx;
+    void* S;                 // source rows: X permuted, then T, then a zero row
+    void* partial;           // far-field column-chunk partial sums
+    const int32_t* tmap0;    // permuted target row -> 0-based original row
+    const int32_t* inv_smap; // 0-based original source row -> permuted row
+    const void* X;
+    void* Y;
+    int64_t ldx, ldy;
+    int64_t n;               // source rows (S rows before T)
+    uint32_t s_bytes;        // bytes of S
+    int32_t nrhs;            // right-hand sides of the call
+    int32_t rhs0;            // first right-hand side of this pass
+    int32_t pad;
+};
+
+// ---- element conversions ---------------------------------------------------
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+template <typename T>
+__device__ inline T from_double(double v);
+template <>
+__device__ inline double from_double<double>(double v) { return v; }
+template <>
+__device__ inline float from_double<float>(double v) { return static_cast<float>(v); }
+template <>
+__device__ inline bf16s from_double<bf16s>(double v) { return bf16s{f2bf(static_cast<float>(v))}; }
+
+template <typename T>
+__device__ inline T from_float(float v);
+template <>
+__device__ inline double from_float<double>(float v) { return v; }
+template <>
+__device__ inline float from_float<float>(float v) { return v; }
+template <>
+__device__ inline bf16s from_float<bf16s>(float v) { return bf16s{f2bf(v)}; }
+
+// ---- storage traits ----------------------------------------------------------
+// TK: columns per k-tile (64 bytes per row), EPL: elements per lane chunk.
+template <typename TS>
+struct Store;
+template <>
+struct Store<double> {
+    static constexpr int TK = 8, EPL = 2;
+    using acc = double;
+};
+template <>
+struct Store<float> {
+    static constexpr int TK = 16, EPL = 4;
+    using acc = float;
+};
+template <>
+struct Store<bf16s> {
+    static constexpr int TK = 32, EPL = 8;
+    using acc = float;
+};
+// B-operand (S) element type: W = 16 columns -> MFMA operand type; W = 1, 4 (VALU
+// paths) -> the accumulation type
+template <typename TS, int W>
+struct SType {
+    using type = typename Store<TS>::acc;
+};
+template <>
+struct SType<double, 16> {
+    using type = double;
+};
+template <>
+struct SType<float, 16> {
+    using type = float;
+};
+template <>
+struct SType<bf16s, 16> {
+    using type = bf16s;
+};
+
+constexpr int kWaves = 4;  // waves per workgroup in the panel kernels
+
+// ---- the panel kernel ----------------------------------------------------------
+// Buffer resources (raw, stride 0): loads past num_records return 0, which masks
+// the absent rows of a partial row tile, the padding of the last k-tile and the
+// prefetch past the end of a panel without branches.
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer descriptor
+// Cache policy of the streamed panel loads: non-temporal, so that the matrix
+// stream (read once per product) does not evict the reused source / T rows
+// from L2 (measured: 3-10% faster than the default policy at N = 262,144).
+constexpr int kPanelPolicy = 2;
+constexpr uint32_t kOOB = 0xffffffffu;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
+                                             kRsrcWord3);
+}
+
+template <int N>
+struct IdxV {
+    int32_t v[N];
+};
+
+// Column indices of the lane's EPL columns of k-tile kt.
+template <int EPL, int TK>
+__device__ inline IdxV<EPL> load_idx(__amdgpu_buffer_rsrc_t C, int kt, int lg) {
+    IdxV<EPL> r;
+    const uint32_t off = static_cast<uint32_t>((kt * TK + lg * EPL) * 4);
+    if constexpr (EPL == 2) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(C, off, 0, 0);
+        r.v[0] = static_cast<int32_t>(v[0]);
+        r.v[1] = static_cast<int32_t>(v[1]);
+    } else if constexpr (EPL == 4) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(C, off, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) r.v[e] = static_cast<int32_t>(v[e]);
+    } else {
+        auto v0 = __builtin_amdgcn_raw_buffer_load_b128(C, off, 0, 0);
+        auto v1 = __builtin_amdgcn_raw_buffer_load_b128(C, off + 16, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            r.v[e] = static_cast<int32_t>(v0[e]);
+            r.v[4 + e] = static_cast<int32_t>(v1[e]);
+        }
+    }
+    return r;
+}
+
+template <typename T>
+__device__ inline T buf_load(__amdgpu_buffer_rsrc_t R, uint32_t off);
+template <>
+__device__ inline double buf_load<double>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(R, off, 0, 0);
+    return __builtin_bit_cast(double, v);
+}
+template <>
+__device__ inline float buf_load<float>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(R, off, 0, 0));
+}
+template <>
+__device__ inline bf16s buf_load<bf16s>(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    return bf16s{__builtin_amdgcn_raw_buffer_load_b16(R, off, 0, 0)};
+}
+
+// One S row of W values for the VALU paths (W = 1 or 4).
+template <typename T, int W>
+struct SRow {
+    T v[W];
+};
+template <typename T, int W>
+__device__ inline SRow<T, W> load_srow(__amdgpu_buffer_rsrc_t R, uint32_t off) {
+    SRow<T, W> r;
+    if constexpr (W == 1) {
+        r.v[0] = buf_load<T>(R, off);
+    } else if constexpr (std::is_same<T, float>::value) {
+        auto q = __builtin_amdgcn_raw_buffer_load_b128(R, off, 0, 0);
+        f32x4 f = __builtin_bit_cast(f32x4, q);
+        r.v[0] = f.x; r.v[1] = f.y; r.v[2] = f.z; r.v[3] = f.w;
+    } else {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        d2 lo = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(R, off, 0, 0));
+        d2 hi = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(R, off + 16, 0, 0));
+        r.v[0] = lo.x; r.v[1] = lo.y; r.v[2] = hi.x; r.v[3] = hi.y;
+    }
+    return r;
+}
+
+// One wave's share of a panel: k-tiles kt0, kt0 + kstep, ... of a `rows` x
+// (ktiles * TK) panel at pdata.  B rows come from S: through the column index
+// array (LEAF) or as the contiguous rows col_off + k masked beyond ncols.  The
+// accumulated results are handed to emit(row, col, value), row < 64, col < W.
+// W = 16 runs on MFMA (16 right-hand sides as the N dimension); W = 1 and 4 run
+// on the VALU, lane (row li, k-chunk lg).  Loads are issued one k-tile ahead (A
+// and B) and two ahead (column indices).  CP is the cache policy of the
+// streamed A loads.
+template <typename TS, int W, bool LEAF, int NRT, int CP, typename Emit>
+__device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles, int64_t col_off,
+                                           int ncols, const int32_t* colidx,
+                                           const typename SType<TS, W>::type* S, uint32_t s_bytes,
+                                           int kt0, int kstep, int lane, Emit&& emit) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
+    constexpr uint32_t SB = sizeof(s_t), ROWB = W * sizeof(s_t);
+    const int li = lane & 15, lg = lane >> 4;
+    const uint32_t slab = static_cast<uint32_t>(rows) * 64u;
+    const __amdgpu_buffer_rsrc_t Ar = make_rsrc(pdata, slab * ktiles);
+    const __amdgpu_buffer_rsrc_t Sr = make_rsrc(S, s_bytes);
+    const __amdgpu_buffer_rsrc_t Cr =
+        make_rsrc(LEAF ? colidx + col_off : colidx, LEAF ? ktiles * TK * 4 : 0);
+    uint32_t aoff[NRT];
+    bool aval[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) {
+        const int rr = min(16, rows - 16 * rt);
+        aval[rt] = li < rr;
+        aoff[rt] = static_cast<uint32_t>((16 * rt * TK + (rr * lg + li) * EPL) * sizeof(TS));
+    }
+    const uint32_t bcol = (W == 16 ? li : 0) * SB;
+    using brow_t = typename std::conditional<W == 16, s_t, SRow<s_t, W>>::type;
+
+    struct Stage {
+        u32x4 a[NRT];
+        brow_t b[EPL];
+    };
+    auto load_stage = [&](int kt, const IdxV<EPL>& ix) {
+        Stage st;
+        const uint32_t base = static_cast<uint32_t>(kt) * slab;
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+            const uint32_t o = aval[rt] ? base + aoff[rt] : kOOB;
+            st.a[rt] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(Ar, o, 0, CP));
+        }
+#pragma unroll
+        for (int e = 0; e < EPL; e++) {
+            uint32_t o;
+            if constexpr (LEAF) {
+                o = static_cast<uint32_t>(ix.v[e]) * ROWB + bcol;
+            } else {
+                const int k = kt * TK + lg * EPL + e;
+                o = k < ncols ? static_cast<uint32_t>(col_off + k) * ROWB + bcol : kOOB;
+            }
+            if constexpr (W == 16)
+                st.b[e] = buf_load<s_t>(Sr, o);
+            else
+                st.b[e] = load_srow<s_t, W>(Sr, o);
+        }
+        return st;
+    };
+
+    using vacc = typename std::conditional<W == 16,
+        typename std::conditional<std::is_same<acc_t, double>::value, f64x4, f32x4>::type,
+        SRow<acc_t, W>>::type;
+    vacc acc[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) acc[rt] = vacc{};
+
+    auto compute = [&](const Stage& st) {
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) {
+            const u32x4 raw = st.a[rt];
+            if constexpr (W == 16) {
+                if constexpr (std::is_same<TS, float>::value) {
+                    f32x4 a = __builtin_bit_cast(f32x4, raw);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, st.b[0], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, st.b[1], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, st.b[2], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, st.b[3], acc[rt], 0, 0, 0);
+                } else if constexpr (std::is_same<TS, double>::value) {
+                    typedef double d2 __attribute__((ext_vector_type(2)));
+                    d2 a = __builtin_bit_cast(d2, raw);
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, st.b[0], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, st.b[1], acc[rt], 0, 0, 0);
+                } else {
+                    bf16x8 a = __builtin_bit_cast(bf16x8, raw);
+                    uint16_t bb[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) bb[e] = st.b[e].u;
+                    bf16x8 bv = __builtin_bit_cast(bf16x8, bb);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[rt], 0, 0, 0);
+                }
+            } else {
+                acc_t av[EPL];
+                if constexpr (std::is_same<TS, float>::value) {
+                    f32x4 a = __builtin_bit_cast(f32x4, raw);
+                    av[0] = a.x; av[1] = a.y; av[2] = a.z; av[3] = a.w;
+                } else if constexpr (std::is_same<TS, double>::value) {
+                    typedef double d2 __attribute__((ext_vector_type(2)));
+                    d2 a = __builtin_bit_cast(d2, raw);
+                    av[0] = a.x; av[1] = a.y;
+                } else {
+                    uint16_t h[8];
+                    std::memcpy(h, &raw, 16);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) av[e] = bf2f(h[e]);
+                }
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    acc_t s = acc[rt].v[w];
+#pragma unroll
+                    for (int e = 0; e < EPL; e++) s += av[e] * st.b[e].v[w];
+                    acc[rt].v[w] = s;
+                }
+            }
+        }
+    };
+
+    // Software pipeline, unrolled by two so the stage buffers alternate without
+    // register copies: while stage A is consumed, stage B (next k-tile) and the
+    // column indices of the k-tile after it are in flight.  Loads past the end
+    // of the panel (kt >= ktiles) fall outside the buffer and return zeros;
+    // they are never consumed.
+    int kt = kt0;
+    IdxV<EPL> ixA{}, ixB{};
+    if constexpr (LEAF) {
+        ixA = load_idx<EPL, TK>(Cr, kt, lg);
+        ixB = load_idx<EPL, TK>(Cr, kt + kstep, lg);
+    }
+    Stage sA = load_stage(kt, ixA), sB;
+    while (kt < ktiles) {
+        if constexpr (LEAF) ixA = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
+        sB = load_stage(kt + kstep, ixB);
+        compute(sA);
+        kt += kstep;
+        if (kt >= ktiles) break;
+        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
+        sA = load_stage(kt + kstep, ixA);
+        compute(sB);
+        kt += kstep;
+    }
+
+    if constexpr (W == 16) {
+        // MFMA 16x16 output, column li: f32 accumulators hold rows 4*lg+v of the
+        // row tile, the f64 instruction holds rows 4*v+lg.
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = std::is_same<TS, double>::value ? 4 * v + lg : 4 * lg + v;
+                emit(16 * rt + row, li, static_cast<acc_t>(acc[rt][v]));
+            }
+    } else {
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                acc_t v = acc[rt].v[w];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (lg == 0) emit(16 * rt + li, w, v);
+            }
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void dispatch_nrt(int rows, F&& f) {
+    switch ((rows + 15) >> 4) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    default: f(std::integral_constant<int, 4>{}); break;
+    }
+}
+
+template <bool BATCH>
+__device__ __forceinline__ const InstDesc& inst_of(const InstDesc& one, const InstDesc* insts,
+                                                   int idx) {
+    if constexpr (BATCH)
+        return insts[idx];
+    else
+        return one;
+}
+
+// Leaf panel: the kWaves waves split the k-tiles of one item; partial sums are
+// reduced through LDS and scattered to Y through target_map.
+template <typename TS, int W, int CP, typename TIO, bool BATCH>
+__global__ __launch_bounds__(kWaves * 64) void k_leaf(const Item* __restrict__ items,
+                                                      const InstDesc one,
+                                                      const InstDesc* __restrict__ insts) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    const Item it = items[blockIdx.x];
+    const InstDesc& I = inst_of<BATCH>(one, insts, it.inst);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rows = it.rows;
+    __shared__ acc_t red[kWaves][64][W];
+    auto emit = [&](int row, int j, acc_t v) { red[wave][row][j] = v; };
+    dispatch_nrt(rows, [&](auto nrt) {
+        panel_body<TS, W, true, decltype(nrt)::value, CP>(
+            static_cast<const TS*>(I.panel) + it.data_off, rows, it.ktiles, it.col_off, 0,
+            I.colidx, static_cast<const s_t*>(I.S), I.s_bytes, wave, kWaves, lane, emit);
+    });
+    __syncthreads();
+    TIO* Y = static_cast<TIO*>(I.Y);
+    for (int t = threadIdx.x; t < rows * W; t += kWaves * 64) {
+        const int r = t / W, j = t % W;
+        acc_t s = red[0][r][j];
+#pragma unroll
+        for (int w = 1; w < kWaves; w++) s += red[w][r][j];
+        if (I.rhs0 + j < I.nrhs)
+            Y[static_cast<int64_t>(I.tmap0[it.out_off + r]) + static_cast<int64_t>(I.rhs0 + j) * I.ldy] =
+                static_cast<TIO>(s);
+    }
+}
+
+// Far-field work item: up to kWaves consecutive 64-row groups of the stacked V
+// rows of one source cluster over one column chunk.  Wave w owns row group w
+// and walks every k-tile of the chunk; the waves read the same S rows (the
+// cluster's X rows), so the B operand is fetched once per workgroup and shared
+// through the L1 cache.  No cross-wave reduction is needed.
+struct FarItem {
+    int64_t data_off;  // panel of row group 0; group g at + g*64*ktiles*TK
+    int64_t col_off;   // first S row of the chunk
+    int64_t out_off;   // output row of row 0 of group 0 (group g: + 64*g)
+    int32_t ktiles;
+    int32_t ncols;
+    int32_t rows_last;  // rows of the last group (the others have 64)
+    int16_t nrg;        // number of row groups (<= kWaves)
+    int16_t to_T;       // 1: write T rows of S, 0: write partial rows
+    int32_t inst;
+    int32_t pad;
+};
+
+template <typename TS, int W, int CP, bool BATCH>
+__global__ __launch_bounds__(kWaves * 64) void k_far(const FarItem* __restrict__ items,
+                                                     const InstDesc one,
+                                                     const InstDesc* __restrict__ insts) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    constexpr int TK = Store<TS>::TK;
+    const FarItem it = items[blockIdx.x];
+    const InstDesc& I = inst_of<BATCH>(one, insts, it.inst);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave >= it.nrg) return;
+    const int rows = wave == it.nrg - 1 ? it.rows_last : 64;
+    const TS* pd = static_cast<const TS*>(I.panel) + it.data_off +
+                   static_cast<int64_t>(wave) * 64 * it.ktiles * TK;
+    const int64_t orow = it.out_off + 64 * wave;
+    s_t* S = static_cast<s_t*>(I.S);
+    acc_t* partial = static_cast<acc_t*>(I.partial);
+    auto emit = [&](int row, int j, acc_t v) {
+        if (row >= rows) return;
+        if (it.to_T)
+            S[(I.n + orow + row) * W + j] = from_double<s_t>(static_cast<double>(v));
+        else
+            partial[(orow + row) * W + j] = v;
+    };
+    dispatch_nrt(rows, [&](auto nrt) {
+        panel_body<TS, W, false, decltype(nrt)::value, CP>(pd, rows, it.ktiles, it.col_off,
+                                                           it.ncols, nullptr, S, I.s_bytes, 0, 1,
+                                                           lane, emit);
+    });
+}
+
+// Sum the column-chunk partials of far-field row groups in a fixed order.
+template <typename TS, int W, bool BATCH>
+__global__ __launch_bounds__(256) void k_reduce(const ReduceGroup* __restrict__ groups,
+                                                const InstDesc one,
+                                                const InstDesc* __restrict__ insts) {
+    using acc_t = typename Store<TS>::acc;
+    using s_t = typename SType<TS, W>::type;
+    const ReduceGroup g = groups[blockIdx.x];
+    const InstDesc& I = inst_of<BATCH>(one, insts, g.inst);
+    const acc_t* partial = static_cast<const acc_t*>(I.partial);
+    s_t* S = static_cast<s_t*>(I.S);
+    for (int t = threadIdx.x; t < g.rows * W; t += blockDim.x) {
+        const int r = t / W, j = t % W;
+        acc_t s = 0;
+        for (int c = 0; c < g.nch; c++) s += partial[(g.pbase + static_cast<int64_t>(c) * g.rows + r) * W + j];
+        S[(I.n + g.tbase + r) * W + j] = from_double<s_t>(static_cast<double>(s));
+    }
+}
+
+// S[p][j] = X[smap[p], rhs0 + j]  (zero for j beyond nrhs); thread per original
+// source index, instance = blockIdx.y.
+template <typename TS, int W, typename TIO, bool BATCH>
+__global__ __launch_bounds__(256) void k_gather(const InstDesc one, const InstDesc* __restrict__ insts) {
+    using s_t = typename SType<TS, W>::type;
+    const InstDesc& I = inst_of<BATCH>(one, insts, blockIdx.y);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= I.n) return;
+    const TIO* X = static_cast<const TIO*>(I.X);
+    s_t* S = static_cast<s_t*>(I.S);
+    const int64_t p = I.inv_smap[t];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const int c = I.rhs0 + j;
+        const double v = c < I.nrhs ? static_cast<double>(X[t + static_cast<int64_t>(c) * I.ldx]) : 0.0;
+        S[p * W + j] = from_double<s_t>(v);
+    }
+}
+
+// ---- build-time panel generation ------------------------------------------------
+// Column descriptors of the leaf panels: kind 0 = built-in kernel entry
+// (base = 0-based source point), 1 = U payload, 2 = dense payload (callback
+// kernels), 3 = padding.  Buffer element (row r) = base + r * stride.
+struct ColDesc {
+    int64_t base;
+    int32_t stride;
+    int32_t kind;
+};
+
+__device__ inline double kernel_entry(int kid, const double* a, const double* b) {
+#pragma clang fp contract(off)
+    if (kid == HMX_KERNEL_LAPLACE) {
+        const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        double d2 = dx * dx + dy * dy;
+        d2 = d2 + dz * dz;
+        const double d = sqrt(d2);
+        return d == 0.0 ? 0.0 : 1.0 / d;
+    }
+    // HMX_KERNEL_LOG2D: generic_norm2 branch without scaling (scaling only
+    // triggers for |r| < 1e-154, where the entry is 1.0 anyway).
+    const double r0 = a[0] - b[0], r1 = a[1] - b[1], r2 = a[2] - b[2];
+    double s = r0 * r0;
+    s += r1 * r1;
+    s += r2 * r2;
+    const double d = sqrt(s);
+    return d < 1e-12 ? 1.0 : (-0.5 / 3.141592653589793) * log(d);
+}
+
+// Element (row r, column k) of an item, at tile position q (16-byte chunks).
+template <typename TS>
+__device__ inline void chunk_coords(int q, int rows, int& kt, int& r, int& k0) {
+    constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
+    const int per_slab = rows * 4;
+    kt = q / per_slab;
+    const int c = q % per_slab;
+    const int rt = c / 64;
+    const int rr = min(16, rows - 16 * rt);
+    const int w = c % 64;
+    const int g = w / rr, i = w % rr;
+    r = 16 * rt + i;
+    k0 = kt * TK + g * EPL;
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_gen_leaf(const Item* __restrict__ items, TS* __restrict__ panel,
+                                                  const ColDesc* __restrict__ cols,
+                                                  const int32_t* __restrict__ tmap0,
+                                                  const double* __restrict__ xt,
+                                                  const double* __restrict__ xs,
+                                                  const double* __restrict__ Ubuf,
+                                                  const double* __restrict__ Dbuf, int kid) {
+    constexpr int EPL = Store<TS>::EPL;
+    const Item it = items[blockIdx.x];
+    const int nchunks = it.ktiles * it.rows * 4;
+    for (int q = threadIdx.x; q < nchunks; q += blockDim.x) {
+        int kt, r, k0;
+        chunk_coords<TS>(q, it.rows, kt, r, k0);
+        const int64_t trow = it.out_off + r;
+        const double* a = xt ? xt + 3 * static_cast<int64_t>(tmap0[trow]) : nullptr;
+        TS vals[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; e++) {
+            const ColDesc cd = cols[it.col_off + k0 + e];
+            double v = 0.0;
+            if (cd.kind == 0)
+                v = kernel_entry(kid, a, xs + 3 * cd.base);
+            else if (cd.kind == 1)
+                v = Ubuf[cd.base + static_cast<int64_t>(r) * cd.stride];
+            else if (cd.kind == 2)
+                v = Dbuf[cd.base + static_cast<int64_t>(r) * cd.stride];
+            vals[e] = from_double<TS>(v);
+        }
+        // chunk q of the item sits at data_off + q*EPL (chunk-major tiles)
+        std::memcpy(panel + it.data_off + static_cast<int64_t>(q) * EPL, vals, sizeof(vals));
+    }
+}
+
+// Far-field panels: element (r, k) = V[vrow[vbase + r] + kc0 + k] for k < ncols.
+struct GenA {
+    int64_t vbase;  // first entry of vrow for this item's rows
+    int64_t kc0;    // first column of the chunk inside the source cluster
+};
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_gen_far(const Item* __restrict__ items,
+                                                 const GenA* __restrict__ gens, TS* __restrict__ panel,
+                                                 const int64_t* __restrict__ vrow,
+                                                 const double* __restrict__ Vbuf) {
+    constexpr int EPL = Store<TS>::EPL;
+    const Item it = items[blockIdx.x];
+    const GenA g = gens[blockIdx.x];
+    const int nchunks = it.ktiles * it.rows * 4;
+    for (int q = threadIdx.x; q < nchunks; q += blockDim.x) {
+        int kt, r, k0;
+        chunk_coords<TS>(q, it.rows, kt, r, k0);
+        const int64_t vb = vrow[g.vbase + r] + g.kc0;
+        TS vals[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; e++) {
+            const int k = k0 + e;
+            vals[e] = from_double<TS>(k < it.ncols ? Vbuf[vb + k] : 0.0);
+        }
+        std::memcpy(panel + it.data_off + static_cast<int64_t>(q) * EPL, vals, sizeof(vals));
+    }
+}
+
+}  // namespace hmx

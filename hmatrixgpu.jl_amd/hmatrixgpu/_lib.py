@@ -57,6 +57,14 @@ SIGNATURES = {
     "hmx_hmatrix_destroy": (C.c_int, [vp]),
     "hmx_matvec": (C.c_int, [vp, C.c_int32, vp, C.c_int64, vp, C.c_int64, C.c_int32, C.c_int32,
                              vp]),
+    "hmx_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.POINTER(vp)]),
+    "hmx_batch_matvec": (C.c_int, [vp, C.POINTER(vp), i64p, C.POINTER(vp), i64p, vp]),
+    "hmx_batch_autotune": (C.c_int, [vp, C.POINTER(vp), i64p, C.POINTER(vp), i64p, C.c_int32, vp,
+                                     C.POINTER(C.c_double)]),
+    "hmx_batch_set_config": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "hmx_batch_get_config": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "hmx_batch_destroy": (C.c_int, [vp]),
     "hmx_profile_enable": (C.c_int, [vp, C.c_int32]),
     "hmx_profile_read": (C.c_int, [vp, f64p, i64p, C.c_int32]),
 }

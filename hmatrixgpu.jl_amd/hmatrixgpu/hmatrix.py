@@ -311,6 +311,14 @@ def _torch():
     return torch
 
 
+def _ld(t):
+    """Leading dimension of a vector or column-major matrix tensor.  A single
+    column's stride is meaningless (torch normalises it), so it is the row count."""
+    if t.dim() == 1 or t.shape[1] == 1:
+        return t.shape[0]
+    return t.stride(1)
+
+
 def mul_(result, H, x, storage=None):
     """mul!(result, H, x) (mult.jl:159-189): result[:, c] = H * x[:, c].
 
@@ -342,14 +350,13 @@ def mul_(result, H, x, storage=None):
         nrhs = x.shape[1]
         if x.stride(0) != 1 or result.stride(0) != 1:
             raise ValueError("mul_: matrices must be column-major (stride(0) == 1)")
-        ldx, ldy = x.stride(1), result.stride(1)
+        ldx, ldy = _ld(x), _ld(result)
         if result.shape[1] != nrhs:
             raise ValueError("mul_: column count mismatch")
     if x.shape[0] != H.n or result.shape[0] != H.m:
         raise ValueError("mul_: size mismatch (H is %d x %d)" % (H.m, H.n))
     stream = torch.cuda.current_stream(x.device).cuda_stream
-    H.matvec_ptr(x.data_ptr(), max(ldx, H.n), result.data_ptr(), max(ldy, H.m), nrhs, io,
-                 stream, storage)
+    H.matvec_ptr(x.data_ptr(), ldx, result.data_ptr(), ldy, nrhs, io, stream, storage)
     return result
 
 
@@ -364,6 +371,79 @@ def product(H, x, storage=None):
     else:
         y = torch.empty((x.shape[1], H.m), dtype=x.dtype, device=x.device).t()
     return mul_(y, H, x, storage=storage)
+
+
+class HMatrixBatch:
+    """Many independent H-matrices multiplied in one captured hipGraph launch
+    (hmx_batch_*).  members must be materialised in `storage` on one device and
+    outlive the batch.  Operands are lists of torch CUDA tensors (vectors or
+    column-major n_i x nrhs matrices), one per member."""
+
+    ORDERS = ("lpt", "tree", "xcd")
+
+    def __init__(self, members, storage="f32", nrhs=4, io="f32"):
+        self.members = list(members)
+        self.storage = STORAGE[storage]
+        self.nrhs = int(nrhs)
+        self.io = L.HMX_F64 if io in ("f64", "float64") else L.HMX_F32
+        arr = (L.vp * len(self.members))(*[m._h for m in self.members])
+        h = L.vp()
+        L.check(L.lib.hmx_batch_create(arr, len(self.members), self.storage, self.nrhs, self.io,
+                                       C.byref(h)))
+        self._h = h
+
+    def _operands(self, Ys, Xs):
+        n = len(self.members)
+        if len(Xs) != n or len(Ys) != n:
+            raise ValueError("one X and one Y per member")
+        xp = (L.vp * n)(*[x.data_ptr() for x in Xs])
+        yp = (L.vp * n)(*[y.data_ptr() for y in Ys])
+        lx = np.array([_ld(x) for x in Xs], np.int64)
+        ly = np.array([_ld(y) for y in Ys], np.int64)
+        torch = _torch()
+        want = torch.float64 if self.io == L.HMX_F64 else torch.float32
+        for m, x, y in zip(self.members, Xs, Ys):
+            cols = [t.shape[1] if t.dim() == 2 else 1 for t in (x, y)]
+            if x.shape[0] != m.n or y.shape[0] != m.m or cols != [self.nrhs] * 2:
+                raise ValueError("batch operand shape mismatch")
+            if x.dtype != want or y.dtype != want or not (x.is_cuda and y.is_cuda):
+                raise TypeError("batch operands must be %s device tensors" % want)
+            if x.stride(0) != 1 or y.stride(0) != 1:
+                raise ValueError("batch operands must be column-major (stride(0) == 1)")
+        return xp, lx, yp, ly
+
+    def matvec(self, Ys, Xs):
+        torch = _torch()
+        xp, lx, yp, ly = self._operands(Ys, Xs)
+        stream = torch.cuda.current_stream(Xs[0].device).cuda_stream
+        L.check(L.lib.hmx_batch_matvec(self._h, xp, L.ptr_i64(lx), yp, L.ptr_i64(ly),
+                                       C.c_void_p(stream)))
+        return Ys
+
+    def autotune(self, Ys, Xs, iters=5):
+        torch = _torch()
+        xp, lx, yp, ly = self._operands(Ys, Xs)
+        stream = torch.cuda.current_stream(Xs[0].device).cuda_stream
+        best = C.c_double()
+        L.check(L.lib.hmx_batch_autotune(self._h, xp, L.ptr_i64(lx), yp, L.ptr_i64(ly), int(iters),
+                                         C.c_void_p(stream), C.byref(best)))
+        return best.value
+
+    def config(self):
+        w, o = C.c_int32(), C.c_int32()
+        L.check(L.lib.hmx_batch_get_config(self._h, C.byref(w), C.byref(o)))
+        return w.value, self.ORDERS[o.value]
+
+    def set_config(self, rhs_width, order="lpt"):
+        L.check(L.lib.hmx_batch_set_config(self._h, int(rhs_width), self.ORDERS.index(order)))
+
+    _destroy = L.lib.hmx_batch_destroy
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._destroy(h)
+            self._h = None
 
 
 # ---- backend registry (HMatrixGPU.jl:14-109) ------------------------------------

@@ -161,6 +161,29 @@ int hmx_hmatrix_destroy(hmx_hmatrix* h);
 int hmx_matvec(hmx_hmatrix* h, int32_t storage, const void* X, int64_t ldx, void* Y, int64_t ldy,
                int32_t nrhs, int32_t io_dtype, void* stream);
 
+/* ---- batched products of independent H-matrices (no reference counterpart:
+ *      the reference runs one mul! per H-matrix, mult.jl:159-189) -------------
+ * All members must be materialised in `storage` on one device; they must stay
+ * alive while the batch exists.  Every stage of the product is one kernel over
+ * all members, and the four stages are replayed as one captured hipGraph per
+ * call.  1 <= nrhs <= 16 (the same for all members). */
+typedef struct hmx_batch hmx_batch;
+int hmx_batch_create(hmx_hmatrix* const* members, int32_t count, int32_t storage, int32_t nrhs,
+                     int32_t io_dtype, hmx_batch** out);
+/* X[i], Y[i]: device pointers of member i (column-major, leading dims ldx[i],
+ * ldy[i]; NULL ld arrays mean n_i / m_i).  Asynchronous on `stream`. */
+int hmx_batch_matvec(hmx_batch* b, const void* const* X, const int64_t* ldx, void* const* Y,
+                     const int64_t* ldy, void* stream);
+/* Autotuner: times every candidate (RHS path: 4-wide VALU or 16-wide MFMA when
+ * nrhs <= 4; leaf dispatch order: 0 largest first, 1 tree, 2 XCD chunks) on the
+ * given operands, keeps the fastest, appends it to the file named by
+ * HMX_TUNE_CACHE (read back by hmx_batch_create).  best_ms may be NULL. */
+int hmx_batch_autotune(hmx_batch* b, const void* const* X, const int64_t* ldx, void* const* Y,
+                       const int64_t* ldy, int32_t iters, void* stream, double* best_ms);
+int hmx_batch_set_config(hmx_batch* b, int32_t rhs_width, int32_t order);
+int hmx_batch_get_config(const hmx_batch* b, int32_t* rhs_width, int32_t* order);
+int hmx_batch_destroy(hmx_batch* b);
+
 /* Per-stage kernel timing with HIP events recorded on the launch stream
  * (stage 0 gather, 1 far-field V*X, 2 partial reduce, 3 near+far U panel
  * product).  enable != 0 starts recording; read returns summed milliseconds

@@ -23,3 +23,17 @@ def rel(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def box_instances(count, nmin, nmax, seed):
+    """Uneven instances: N log-uniform in [nmin, nmax], points uniform in a box whose
+    side along one random axis is a ~ U[1, 8] (the others 1).  Returns a list of
+    (3, N) arrays."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))))
+        side = np.ones(3)
+        side[rng.integers(3)] = rng.uniform(1.0, 8.0)
+        out.append((rng.random((n, 3)) * side).T.copy())
+    return out

@@ -124,3 +124,14 @@ def test_hmatrixcpu_fields():
     for (r0, r1, c0, c1), U, V in zip(hc.approx_block_indices, hc.U_matrices, hc.V_matrices):
         assert U.shape[0] == r1 - r0 + 1 and V.shape[1] == c1 - c0 + 1
         assert U.shape[1] == V.shape[0]
+
+
+def test_batch_create_rejects_bad_arguments():
+    import ctypes as C
+    import hmatrixgpu._lib as L
+    out = L.vp()
+    assert L.lib.hmx_batch_create(None, 0, 1, 4, 1, C.byref(out)) == 1
+    arr = (L.vp * 1)(None)
+    assert L.lib.hmx_batch_create(arr, 1, 1, 17, 1, C.byref(out)) == 1
+    assert L.lib.hmx_batch_create(arr, 1, 1, 4, 1, C.byref(out)) == 1
+    assert b"NULL" in L.lib.hmx_last_error()
