@@ -1,0 +1,429 @@
+// aca_gpu.hip -- batched FP64 ACA+ on the GPU with a rank capacity per block.
+//
+// Restates ACA_plus (aca.jl:36-106) for built-in kernels: one 256-thread
+// workgroup per admissible block runs the adaptive cross approximation with
+// the reference's pivot rules (aca.jl:65-87), residual order (aca.jl:43-57),
+// term normalisation (aca.jl:92-93) and stop test (aca.jl:95-99).  Every
+// element of a residual row / column is computed by one thread in the same
+// operation order as the host restatement (contraction off), the pivot search
+// is a total-order argmax (largest |r|, lowest index on ties), and the stop
+// test's sums of squares use the fixed 256-lane order of sumsq_lanes, so the
+// factors are bit-identical to the host ACA+.
+//
+// Each block gets term slots for `cap` terms (the rank capacity, e.g. from the
+// rank predictor).  A block that reaches its capacity without meeting the stop
+// test is not finished: its state (terms, last residual row and column, used
+// pivots) is kept and it continues in the next pass with a doubled capacity.
+// Because ACA+ is sequential and deterministic, capping and continuing gives
+// exactly the factors of an uncapped run; the capacity only changes the
+// schedule (passes, launches) and the padding (slots never filled).
+#include "device_op.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+
+namespace hmx {
+
+namespace {
+
+constexpr int kAcaThreads = 256;
+static_assert(kAcaThreads == kSumLanes, "stop-test sum order is defined over 256 lanes");
+
+struct AcaTask {
+    int64_t t0, s0;  // first permuted target row / source column
+    int64_t uoff;    // term q of U at U + uoff + q*m
+    int64_t voff;    // term q of V at V + voff + q*n
+    int64_t woff;    // residual row (n) then residual column (m) at work + woff
+    int64_t foff;    // used-column (n) then used-row (m) flags at flags + foff
+    int32_t m, n;
+    int32_t kcap;    // term slots of this pass
+    int32_t kmax;    // min(m, n)  (aca.jl:60 with max_iter = n_rows)
+    int32_t sid;     // state index
+    int32_t pad;
+};
+
+struct AcaState {
+    int32_t k;
+    int32_t done;
+};
+
+struct Best {
+    double v;
+    int i;
+};
+
+// (vb, ib) replaces (va, ia): larger value under isless (NaN largest), lower
+// index on ties; index -1 is "none".
+__device__ __forceinline__ bool better(double vb, int ib, double va, int ia) {
+    if (ib < 0) return false;
+    if (ia < 0) return true;
+    const bool na = isnan(va), nb = isnan(vb);
+    if (na || nb) return na && nb ? ib < ia : nb;
+    if (vb != va) return vb > va;
+    return ib < ia;
+}
+
+// argmax_not_in_list(abs.(r), used) (aca.jl:9-17) over len entries.
+__device__ Best block_argmax(const double* r, const uint8_t* used, int len, Best* lds) {
+    Best b{0.0, -1};
+    for (int j = threadIdx.x; j < len; j += kAcaThreads) {
+        if (used[j]) continue;
+        const double a = fabs(r[j]);
+        if (better(a, j, b.v, b.i)) b = Best{a, j};
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(b.v, off);
+        const int oi = __shfl_xor(b.i, off);
+        if (better(ov, oi, b.v, b.i)) b = Best{ov, oi};
+    }
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = b;
+    __syncthreads();
+    Best r0 = lds[0];
+#pragma unroll
+    for (int w = 1; w < kAcaThreads / 64; w++)
+        if (better(lds[w].v, lds[w].i, r0.v, r0.i)) r0 = lds[w];
+    __syncthreads();
+    return r0;
+}
+
+// Fold the 256 lane partials: p[l] += p[l + off], off = 128 ... 1.
+__device__ void tree_fold2(double* pa, double* pb) {
+#pragma clang fp contract(off)
+    for (int off = kAcaThreads / 2; off >= 1; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            pa[threadIdx.x] = pa[threadIdx.x] + pa[threadIdx.x + off];
+            pb[threadIdx.x] = pb[threadIdx.x] + pb[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaTask* __restrict__ tasks,
+                                                     AcaState* __restrict__ states,
+                                                     const double* __restrict__ xt,
+                                                     const double* __restrict__ xs,
+                                                     double* U, double* V,
+                                                     double* __restrict__ work,
+                                                     uint8_t* __restrict__ flags, int kid,
+                                                     double eps) {
+#pragma clang fp contract(off)
+    const AcaTask T = tasks[blockIdx.x];
+    const AcaState st0 = states[T.sid];
+    if (st0.done) return;
+    const int m = T.m, n = T.n, tid = threadIdx.x;
+    double* rrow = work + T.woff;
+    double* rcol = rrow + n;
+    uint8_t* usedc = flags + T.foff;
+    uint8_t* usedr = usedc + n;
+    double* Ub = U + T.uoff;
+    double* Vb = V + T.voff;
+    __shared__ Best red[kAcaThreads / 64];
+    __shared__ double pu[kAcaThreads], pv[kAcaThreads];
+    int k = st0.k;
+    int done = 0;
+
+    // calc_residual_rows / calc_residual_cols (aca.jl:43-57)
+    auto residual_row = [&](int I) {
+        const double* a = xt + 3 * (T.t0 + I);
+        for (int j = tid; j < n; j += kAcaThreads) {
+            double e = kernel_entry(kid, a, xs + 3 * (T.s0 + j));
+            for (int q = 0; q < k; q++)
+                e = e - Ub[static_cast<int64_t>(q) * m + I] * Vb[static_cast<int64_t>(q) * n + j];
+            rrow[j] = e;
+        }
+        __syncthreads();
+    };
+    auto residual_col = [&](int J) {
+        const double* b = xs + 3 * (T.s0 + J);
+        for (int i = tid; i < m; i += kAcaThreads) {
+            double e = kernel_entry(kid, xt + 3 * (T.t0 + i), b);
+            for (int q = 0; q < k; q++)
+                e = e - Ub[static_cast<int64_t>(q) * m + i] * Vb[static_cast<int64_t>(q) * n + J];
+            rcol[i] = e;
+        }
+        __syncthreads();
+    };
+
+    while (k < T.kmax && k < T.kcap) {
+        int rp = 0, cp = 0;
+        double rval = 0.0, cval = 0.0;  // first step: pivot (1,1) of a zero residual
+        if (k > 0) {
+            const Best bc = block_argmax(rrow, usedc, n, red);
+            const Best br = block_argmax(rcol, usedr, m, red);
+            cp = bc.i;
+            cval = bc.v;
+            rp = br.i;
+            rval = br.v;
+        }
+        if (rval > cval) {  // aca.jl:79-87
+            residual_row(rp);
+            cp = block_argmax(rrow, usedc, n, red).i;
+            residual_col(cp);
+        } else {
+            residual_col(cp);
+            rp = block_argmax(rcol, usedr, m, red).i;
+            residual_row(rp);
+        }
+        const double piv = rrow[cp];
+        double su = 0.0, sv = 0.0;
+        for (int j = tid; j < n; j += kAcaThreads) {  // aca.jl:92
+            const double v = rrow[j] / piv;
+            Vb[static_cast<int64_t>(k) * n + j] = v;
+            const double sq = v * v;
+            sv = sv + sq;
+        }
+        for (int i = tid; i < m; i += kAcaThreads) {  // aca.jl:93
+            const double u = rcol[i];
+            Ub[static_cast<int64_t>(k) * m + i] = u;
+            const double sq = u * u;
+            su = su + sq;
+        }
+        pu[tid] = su;
+        pv[tid] = sv;
+        if (tid == 0) {
+            usedr[rp] = 1;
+            usedc[cp] = 1;
+        }
+        __syncthreads();
+        tree_fold2(pu, pv);
+        const double step = sqrt(pu[0] * pv[0]);  // aca.jl:95
+        __syncthreads();
+        k++;
+        if (step < eps) {  // aca.jl:97-99
+            done = 1;
+            break;
+        }
+    }
+    if (k >= T.kmax) done = 1;
+    if (tid == 0) states[T.sid] = AcaState{k, done};
+}
+
+// Copy the first `len` doubles of each segment (continuation of a block into
+// a larger slot).  One workgroup per segment.
+struct Seg {
+    int64_t src, dst, len;
+};
+
+__global__ __launch_bounds__(256) void k_copy_segs(const Seg* __restrict__ segs,
+                                                   const double* __restrict__ from,
+                                                   double* __restrict__ to) {
+    const Seg s = segs[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < s.len; i += 256) to[s.dst + i] = from[s.src + i];
+}
+
+int64_t round_cap(int64_t c, int64_t kmax) {
+    c = std::max<int64_t>(4, (c + 3) / 4 * 4);
+    return std::min(c, kmax);
+}
+
+}  // namespace
+
+int64_t aca_round_cap(int64_t c, int64_t kmax) { return round_cap(c, kmax); }
+
+void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
+             const std::vector<int32_t>& caps, double eps, int nthreads, AcaStats* stats,
+             const std::function<void(int64_t, LowRank&&)>& on_done) {
+    // 1/r only: its entry (subtractions, products, sqrt, division) is correctly
+    // rounded on both sides, so the device reproduces the host factors exactly;
+    // the log kernel's device log() is not guaranteed to match the host's.
+    if (h.src.kernel_id != HMX_KERNEL_LAPLACE)
+        throw Error(HMX_ERR_UNSUPPORTED, "GPU ACA+ supports the built-in 1/r kernel");
+    if (h.target_dims != 1 || h.source_dims != 1)
+        throw Error(HMX_ERR_UNSUPPORTED, "GPU ACA+ needs scalar kernels (dims == 1)");
+    DeviceGuard guard(h.device);
+    AcaStats S{};
+    const auto wall0 = std::chrono::steady_clock::now();
+
+    // permuted coordinates: point p of the permuted order = original tmap[p]-1
+    std::vector<double> xtp(3 * h.m), xsp(3 * h.n);
+    for (int64_t p = 0; p < h.m; p++)
+        for (int c = 0; c < 3; c++) xtp[3 * p + c] = h.src.xt[3 * (h.tmap[p] - 1) + c];
+    for (int64_t p = 0; p < h.n; p++)
+        for (int c = 0; c < 3; c++) xsp[3 * p + c] = h.src.xs[3 * (h.smap[p] - 1) + c];
+    double* dxt = upload(xtp);
+    double* dxs = upload(xsp);
+    std::vector<void*> owned = {dxt, dxs};
+    auto release = [&](void* p) {
+        if (!p) return;
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it != owned.end()) owned.erase(it);
+        (void)hipFree(p);
+    };
+    auto free_all = [&] {
+        for (void* p : owned)
+            if (p) (void)hipFree(p);
+        owned.clear();
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        const int64_t nb = static_cast<int64_t>(blocks.size());
+        // most expensive first: cost ~ (m + n) * cap^2
+        std::vector<int64_t> order(nb);
+        for (int64_t b = 0; b < nb; b++) order[b] = b;
+        auto cost = [&](int64_t b) {
+            const double c = caps[b];
+            return double(blocks[b].m + blocks[b].n) * c * c;
+        };
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+        const char* env = std::getenv("HMX_ACA_CHUNK_BYTES");
+        const int64_t budget = env ? std::atoll(env) : (int64_t(8) << 30);
+        int64_t pos = 0;
+        while (pos < nb) {
+            // one chunk: term slots + residual workspace within the budget
+            int64_t end = pos, bytes = 0;
+            while (end < nb) {
+                const AcaBlock& B = blocks[order[end]];
+                const int64_t c = round_cap(caps[order[end]], std::min(B.m, B.n));
+                const int64_t add = (B.m + B.n) * (8 * c + 9);
+                if (end > pos && bytes + add > budget) break;
+                bytes += add;
+                end++;
+            }
+            const int64_t nc = end - pos;
+            std::vector<AcaTask> tasks(nc);
+            int64_t wtot = 0;
+            for (int64_t t = 0; t < nc; t++) {
+                const AcaBlock& B = blocks[order[pos + t]];
+                AcaTask& a = tasks[t];
+                a.t0 = B.t0;
+                a.s0 = B.s0;
+                a.m = static_cast<int32_t>(B.m);
+                a.n = static_cast<int32_t>(B.n);
+                a.kmax = static_cast<int32_t>(std::min(B.m, B.n));
+                a.kcap = static_cast<int32_t>(round_cap(caps[order[pos + t]], a.kmax));
+                a.sid = static_cast<int32_t>(t);
+                a.woff = wtot;
+                a.foff = wtot;
+                wtot += B.m + B.n;
+            }
+            double* work = nullptr;
+            uint8_t* flags = nullptr;
+            AcaState* dstate = nullptr;
+            HIPCHECK(hipMalloc(&work, std::max<int64_t>(wtot, 1) * sizeof(double)));
+            owned.push_back(work);
+            HIPCHECK(hipMalloc(&flags, std::max<int64_t>(wtot, 1)));
+            owned.push_back(flags);
+            HIPCHECK(hipMalloc(&dstate, nc * sizeof(AcaState)));
+            owned.push_back(dstate);
+            HIPCHECK(hipMemset(work, 0, wtot * sizeof(double)));
+            HIPCHECK(hipMemset(flags, 0, wtot));
+            HIPCHECK(hipMemset(dstate, 0, nc * sizeof(AcaState)));
+            std::vector<AcaState> hstate(nc);
+            std::vector<int64_t> active(nc);
+            for (int64_t t = 0; t < nc; t++) active[t] = t;
+            double* pool = nullptr;  // previous pass's slots (continuing blocks)
+            std::vector<double> host_pool;
+            int pass = 0;
+            while (!active.empty()) {
+                // slots for the active blocks: U terms then V terms
+                int64_t tot = 0;
+                std::vector<Seg> segs;
+                std::vector<int64_t> old_u(active.size()), old_v(active.size());
+                for (size_t a = 0; a < active.size(); a++) {
+                    AcaTask& T = tasks[active[a]];
+                    old_u[a] = T.uoff;
+                    old_v[a] = T.voff;
+                    T.uoff = tot;
+                    tot += int64_t(T.m) * T.kcap;
+                }
+                for (size_t a = 0; a < active.size(); a++) {
+                    AcaTask& T = tasks[active[a]];
+                    T.voff = tot;
+                    tot += int64_t(T.n) * T.kcap;
+                    const int k = hstate[active[a]].k;
+                    if (pass > 0 && k > 0) {
+                        segs.push_back(Seg{old_u[a], T.uoff, int64_t(k) * T.m});
+                        segs.push_back(Seg{old_v[a], T.voff, int64_t(k) * T.n});
+                    }
+                    S.slot_elems += int64_t(T.m + T.n) * T.kcap;
+                }
+                double* npool = nullptr;
+                HIPCHECK(hipMalloc(&npool, std::max<int64_t>(tot, 1) * sizeof(double)));
+                owned.push_back(npool);
+                Seg* dsegs = nullptr;
+                if (!segs.empty()) {
+                    dsegs = upload(segs);
+                    owned.push_back(dsegs);
+                    hipLaunchKernelGGL(k_copy_segs, dim3(segs.size()), dim3(256), 0, nullptr,
+                                       dsegs, pool, npool);
+                    HIPCHECK(hipGetLastError());
+                }
+                std::vector<AcaTask> launch(active.size());
+                for (size_t a = 0; a < active.size(); a++) launch[a] = tasks[active[a]];
+                AcaTask* dt = upload(launch);
+                owned.push_back(dt);
+                HIPCHECK(hipEventRecord(e0, nullptr));
+                hipLaunchKernelGGL(k_aca, dim3(launch.size()), dim3(kAcaThreads), 0, nullptr, dt,
+                                   dstate, dxt, dxs, npool, npool, work, flags, h.src.kernel_id,
+                                   eps);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipEventRecord(e1, nullptr));
+                HIPCHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                S.kernel_ms += ms;
+                S.launches++;
+                HIPCHECK(hipMemcpy(hstate.data(), dstate, nc * sizeof(AcaState),
+                                   hipMemcpyDeviceToHost));
+                host_pool.resize(static_cast<size_t>(tot));
+                HIPCHECK(hipMemcpy(host_pool.data(), npool, tot * sizeof(double),
+                                   hipMemcpyDeviceToHost));
+                // the pass has completed: release its task list, segment list and
+                // the previous pass's slots
+                release(dt);
+                release(dsegs);
+                release(pool);
+                pool = npool;
+                std::vector<int64_t> finished, cont;
+                for (int64_t t : active) (hstate[t].done ? finished : cont).push_back(t);
+                if (pass == 0) S.unconverged_first += static_cast<int64_t>(cont.size());
+                parallel_for(static_cast<int64_t>(finished.size()), nthreads, [&](int64_t f) {
+                    const AcaTask& T = tasks[finished[f]];
+                    const int64_t k = hstate[finished[f]].k;
+                    LowRank L;
+                    L.m = T.m;
+                    L.n = T.n;
+                    L.k = k;
+                    L.U.assign(host_pool.begin() + T.uoff, host_pool.begin() + T.uoff + k * T.m);
+                    L.V.resize(static_cast<size_t>(k * T.n));
+                    const double* src = host_pool.data() + T.voff;
+                    for (int64_t q = 0; q < k; q++)
+                        for (int64_t j = 0; j < T.n; j++) L.V[q + j * k] = src[q * T.n + j];
+                    on_done(order[pos + T.sid], std::move(L));
+                });
+                for (int64_t t : finished) S.used_elems += int64_t(tasks[t].m + tasks[t].n) * hstate[t].k;
+                for (int64_t t : cont) {
+                    AcaTask& T = tasks[t];
+                    T.kcap = static_cast<int32_t>(round_cap(2 * int64_t(T.kcap), T.kmax));
+                }
+                active.swap(cont);
+                pass++;
+            }
+            S.passes = std::max<int64_t>(S.passes, pass);
+            for (void* p : owned)
+                if (p != dxt && p != dxs) (void)hipFree(p);
+            owned.assign({dxt, dxs});
+            S.chunks++;
+            pos = end;
+        }
+        S.blocks = nb;
+    } catch (...) {
+        (void)hipDeviceSynchronize();
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        free_all();
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    free_all();
+    S.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    if (stats) *stats = S;
+}
+
+}  // namespace hmx

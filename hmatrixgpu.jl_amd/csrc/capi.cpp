@@ -8,6 +8,8 @@
 #include <new>
 #include <string>
 
+#include <chrono>
+
 #include "hmx_host.hpp"
 
 struct hmx_tree {
@@ -252,8 +254,35 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
                       const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
                       void* ctx, double eta, double eps, int32_t svd_recompress,
                       int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out) {
+    return hmx_hmatrix_build_ex(target, source, kernel_id, target_xyz, source_xyz, fill, ctx, eta,
+                                eps, svd_recompress, storage_mask, device, nthreads, nullptr,
+                                nullptr, out);
+}
+
+int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                         const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
+                         void* ctx, double eta, double eps, int32_t svd_recompress,
+                         int32_t storage_mask, int32_t device, int32_t nthreads,
+                         const hmx_build_options* options, hmx_build_stats* stats,
+                         hmx_hmatrix** out) {
     return guarded([&] {
         need(target && source && target->t && source->t && out, "hmx_hmatrix_build: NULL argument");
+        hmx::BuildOptions opt;
+        hmx::AcaStats ast;
+        opt.device = device;
+        opt.stats = &ast;
+        if (options) {
+            need(options->aca_device == 0 || options->aca_device == 1,
+                 "hmx_hmatrix_build_ex: aca_device must be 0 (host) or 1 (GPU)");
+            need(options->initial_rank >= 0 && options->cap_margin >= 0.0,
+                 "hmx_hmatrix_build_ex: negative initial_rank or cap_margin");
+            opt.aca_device = options->aca_device;
+            if (options->initial_rank > 0) opt.initial_rank = options->initial_rank;
+            opt.predictor = options->predictor ? options->predictor->p.get() : nullptr;
+            opt.cap_margin = options->cap_margin;
+            if (opt.aca_device == 1 && kernel_id != HMX_KERNEL_LAPLACE)
+                throw hmx::Error(HMX_ERR_UNSUPPORTED, "GPU ACA+ supports the built-in 1/r kernel");
+        }
         *out = nullptr;
         need(storage_mask >= 0 && storage_mask < 8, "hmx_hmatrix_build: bad storage_mask");
         const hmx::ClusterTree& T = *target->t;
@@ -276,7 +305,8 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
             src.xt = xt.data();
             src.xs = xs.data();
         }
-        auto h = hmx::build_hmatrix(T, S, src, eta, eps, svd_recompress != 0, nthreads);
+        const auto t0 = std::chrono::steady_clock::now();
+        auto h = hmx::build_hmatrix(T, S, src, eta, eps, svd_recompress != 0, nthreads, &opt);
         h->xt_copy = std::move(xt);
         h->xs_copy = std::move(xs);
         h->src.xt = h->xt_copy.empty() ? nullptr : h->xt_copy.data();
@@ -284,8 +314,111 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
         h->device = device;
         for (int s = 0; s < 3; s++)
             if (storage_mask & (1 << s)) hmx::device_build(*h, s);
+        if (stats) {
+            stats->blocks = ast.blocks;
+            stats->chunks = ast.chunks;
+            stats->passes = ast.passes;
+            stats->launches = ast.launches;
+            stats->unconverged_first = ast.unconverged_first;
+            stats->slot_elems = ast.slot_elems;
+            stats->used_elems = ast.used_elems;
+            stats->aca_kernel_ms = ast.kernel_ms;
+            stats->aca_wall_s = ast.wall_s;
+            stats->build_s = h->build_seconds;
+            stats->total_s =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
         *out = new hmx_hmatrix{std::move(h)};
     });
+}
+
+int hmx_hmatrix_block_features(const hmx_hmatrix* hm, int64_t* count, float* features,
+                               int32_t* aca_rank, int32_t* rank, int32_t* capacity) {
+    return guarded([&] {
+        need(hm && hm->h && count, "hmx_hmatrix_block_features: NULL");
+        const auto& st = hm->h->approx_stats;
+        *count = static_cast<int64_t>(st.size());
+        for (size_t b = 0; b < st.size(); b++) {
+            if (features)
+                for (int i = 0; i < hmx::kRankFeatures; i++)
+                    features[b * hmx::kRankFeatures + i] = st[b].feat[i];
+            if (aca_rank) aca_rank[b] = st[b].aca_rank;
+            if (rank) rank[b] = st[b].rank;
+            if (capacity) capacity[b] = st[b].cap;
+        }
+    });
+}
+
+// ---- rank predictor ---------------------------------------------------------------
+namespace {
+std::vector<float> rank_targets(const float* ranks, int64_t n) {
+    std::vector<float> z(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; i++) {
+        need(ranks[i] >= 0.f, "rank predictor: negative rank");
+        z[i] = static_cast<float>(std::log1p(static_cast<double>(ranks[i])));
+    }
+    return z;
+}
+}  // namespace
+
+int hmx_rankpred_create(int32_t hidden, uint64_t seed, int32_t device, hmx_rankpred** out) {
+    return guarded([&] {
+        need(out != nullptr, "hmx_rankpred_create: NULL");
+        *out = nullptr;
+        auto* p = hmx::rankpred_create(hidden, seed, device);
+        *out = new hmx_rankpred{std::unique_ptr<hmx::RankPredictor, hmx_rankpred::Del>(p)};
+    });
+}
+
+int hmx_rankpred_nparams(int64_t* n) {
+    return guarded([&] {
+        need(n != nullptr, "hmx_rankpred_nparams: NULL");
+        *n = hmx::rankpred_nparams();
+    });
+}
+
+int hmx_rankpred_train(hmx_rankpred* p, const float* features, const float* ranks, int64_t n,
+                       int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                       float* loss_per_epoch) {
+    return guarded([&] {
+        need(p && p->p && features && ranks, "hmx_rankpred_train: NULL");
+        need(n > 0 && n < (int64_t(1) << 31) && epochs > 0 && batch > 0 && lr > 0.f,
+             "hmx_rankpred_train: bad size, epochs, batch or learning rate");
+        const auto z = rank_targets(ranks, n);
+        hmx::rankpred_train(*p->p, features, z.data(), n, epochs, batch, lr, seed, loss_per_epoch);
+    });
+}
+
+int hmx_rankpred_grad(hmx_rankpred* p, const float* features, const float* ranks, int64_t n,
+                      float* grad, float* loss) {
+    return guarded([&] {
+        need(p && p->p && features && ranks && grad, "hmx_rankpred_grad: NULL");
+        need(n > 0 && n < (int64_t(1) << 31), "hmx_rankpred_grad: bad size");
+        const auto z = rank_targets(ranks, n);
+        hmx::rankpred_grad(*p->p, features, z.data(), n, grad, loss);
+    });
+}
+
+int hmx_rankpred_predict(const hmx_rankpred* p, const float* features, int64_t n, float* ranks) {
+    return guarded([&] {
+        need(p && p->p && features && ranks, "hmx_rankpred_predict: NULL");
+        need(n >= 0 && n < (int64_t(1) << 31), "hmx_rankpred_predict: bad size");
+        hmx::rankpred_predict(*p->p, features, n, ranks);
+        for (int64_t i = 0; i < n; i++) ranks[i] = static_cast<float>(std::expm1(double(ranks[i])));
+    });
+}
+
+int hmx_rankpred_params(hmx_rankpred* p, int32_t get, float* master, float* bf16, float* norm) {
+    return guarded([&] {
+        need(p && p->p, "hmx_rankpred_params: NULL");
+        need(get || !bf16, "hmx_rankpred_params: the BF16 copy is derived, set the master weights");
+        hmx::rankpred_params(*p->p, master, bf16, norm, get != 0);
+    });
+}
+
+int hmx_rankpred_destroy(hmx_rankpred* p) {
+    delete p;
+    return HMX_OK;
 }
 
 int hmx_hmatrix_materialize(hmx_hmatrix* hm, int32_t storage_mask, int32_t device) {

@@ -25,6 +25,8 @@ struct Error : std::runtime_error {
 };
 
 // ---- cluster tree (tree.jl:6-20) ------------------------------------------
+struct BuildOptions;
+
 struct ClusterNode {
     int64_t start;  // 1-based
     int64_t end;    // 1-based, exclusive
@@ -95,6 +97,9 @@ using RowFn = std::function<void(int64_t, double*)>;
 LowRank aca_plus(int64_t m, int64_t n, const RowFn& row, const RowFn& col, double eps,
                  int64_t max_iter);
 LowRank svd_recompress(const LowRank& in, double eps);
+// Fixed-order sum of squares used by the ACA+ stop test (host and device).
+constexpr int kSumLanes = 256;
+double sumsq_lanes(const double* x, int64_t n);
 
 // ---- H-matrix -----------------------------------------------------------------
 struct DenseBlock {
@@ -132,12 +137,21 @@ struct HMatrix {
     int32_t device = 0;
     int64_t target_dims = 1, source_dims = 1;
     std::unique_ptr<DeviceOp, DeviceOpDeleter> dev[3];
+    // per admissible block, approx-list order (hmatrix.jl:189-213)
+    struct ApproxStat {
+        float feat[6];     // log diam_s, log diam_t, log dist, log n_s, log n_t, log tol
+        int32_t aca_rank;  // ACA+ terms before recompression
+        int32_t rank;      // rank after recompression
+        int32_t cap;       // first rank capacity of the device ACA+ (0: host ACA+)
+    };
+    std::vector<ApproxStat> approx_stats;
     void ensure_dense_payload();
 };
 
 std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& S,
                                        const EntrySource& src, double eta, double eps,
-                                       bool svd, int nthreads);
+                                       bool svd, int nthreads,
+                                       const BuildOptions* opt = nullptr);
 
 // Simple fixed-size worker pool: run f(i) for i in [0,n) on `threads` threads.
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f);
@@ -152,10 +166,68 @@ int64_t device_index_bytes(const DeviceOp* op);
 void device_profile_enable(HMatrix& h, bool on);
 void device_profile_read(HMatrix& h, double* ms, int64_t* launches, bool reset);
 
+// ---- batched ACA+ on the device (aca_gpu.hip) ------------------------------------
+struct AcaBlock {
+    int64_t t0, s0;  // 0-based first row / column in the permuted index spaces
+    int64_t m, n;
+};
+struct AcaStats {
+    int64_t blocks = 0;
+    int64_t chunks = 0;             // memory-bounded groups of blocks
+    int64_t passes = 0;             // most capacity passes any chunk needed
+    int64_t launches = 0;           // ACA+ kernel launches
+    int64_t unconverged_first = 0;  // blocks that outgrew their first capacity
+    int64_t slot_elems = 0;         // sum over passes of (m + n) * capacity
+    int64_t used_elems = 0;         // sum of (m + n) * final rank
+    double kernel_ms = 0.0;         // ACA+ kernel time (HIP events)
+    double wall_s = 0.0;            // whole phase including copies and callbacks
+};
+// ACA+ (eps is the ACA tolerance) of every block with the given initial rank
+// capacities; on_done(b, factors) is called from host worker threads as blocks
+// finish.  Built-in scalar kernels only.
+void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
+             const std::vector<int32_t>& caps, double eps, int nthreads, AcaStats* stats,
+             const std::function<void(int64_t, LowRank&&)>& on_done);
+// capacity rounding of a pass (multiple of 4, at least 4, at most kmax)
+int64_t aca_round_cap(int64_t c, int64_t kmax);
+
+// ---- rank predictor (rankpred.hip) ----------------------------------------------
+// 6 -> 64 -> 64 -> 1 MLP on (log diam_s, log diam_t, log dist, log n_s, log n_t,
+// log tol) -> z = log(1 + ACA+ rank).  Rows of X are samples (6 floats).
+struct RankPredictor;
+RankPredictor* rankpred_create(int hidden, uint64_t seed, int device);
+void rankpred_destroy(RankPredictor* p);
+int64_t rankpred_nparams();
+void rankpred_train(RankPredictor& p, const float* X, const float* Z, int64_t n, int epochs,
+                    int batch, float lr, uint64_t seed, float* loss_hist);
+void rankpred_grad(RankPredictor& p, const float* X, const float* Z, int64_t n, float* grad,
+                   float* loss);
+void rankpred_predict(const RankPredictor& p, const float* X, int64_t n, float* z_out);
+void rankpred_params(RankPredictor& p, float* master, float* bf16, float* norm, bool get);
+constexpr int kRankFeatures = 6;
+
+// Options of the H-matrix build beyond the reference's keywords.
+struct BuildOptions {
+    int aca_device = 0;                       // 0 host threads, 1 batched ACA+ on the GPU
+    int device = 0;                           // GPU ordinal of the device ACA+
+    int initial_rank = 16;                    // first capacity without a predictor
+    const RankPredictor* predictor = nullptr;  // capacity from the predicted rank
+    double cap_margin = 0.25;                 // capacity = (1 + margin) * predicted rank
+    AcaStats* stats = nullptr;
+};
+
 // Thread-local message returned by hmx_last_error().
 void set_last_error(const char* msg);
 
 }  // namespace hmx
+
+// Opaque ABI handle of a rank predictor (include/hmx.h).
+struct hmx_rankpred {
+    struct Del {
+        void operator()(hmx::RankPredictor* p) const { hmx::rankpred_destroy(p); }
+    };
+    std::unique_ptr<hmx::RankPredictor, Del> p;
+};
 
 // Opaque ABI handle of an H-matrix (include/hmx.h).
 struct hmx_hmatrix {

@@ -112,9 +112,28 @@ void HMatrix::ensure_dense_payload() {
     D_ready = true;
 }
 
+namespace {
+
+// Rank-predictor inputs of an admissible block (rankpred.hip).
+void block_features(const ClusterNode& t, const ClusterNode& s, double tol, float* f) {
+    auto lg = [](double x) { return static_cast<float>(std::log(std::max(x, 1e-12))); };
+    const double d[3] = {t.center[0] - s.center[0], t.center[1] - s.center[1],
+                         t.center[2] - s.center[2]};
+    f[0] = lg(2.0 * s.radius);
+    f[1] = lg(2.0 * t.radius);
+    f[2] = lg(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    f[3] = lg(static_cast<double>(s.end - s.start));
+    f[4] = lg(static_cast<double>(t.end - t.start));
+    f[5] = lg(tol);
+}
+
+}  // namespace
+
 std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& S,
                                        const EntrySource& src, double eta, double eps, bool svd,
-                                       int nthreads) {
+                                       int nthreads, const BuildOptions* opt) {
+    const BuildOptions defaults;
+    if (!opt) opt = &defaults;
     auto t0 = std::chrono::steady_clock::now();
     auto h = std::make_unique<HMatrix>();
     h->src = src;
@@ -131,9 +150,47 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
     traverse(*bt, &direct, &approx);        // :78
     bt.reset();
 
+    const int64_t na = static_cast<int64_t>(approx.size());
+    h->approx_stats.assign(static_cast<size_t>(na), HMatrix::ApproxStat{});
+    for (int64_t b = 0; b < na; b++)
+        block_features(T.nodes[approx[b].first], S.nodes[approx[b].second], eps,
+                       h->approx_stats[b].feat);
+
     // Compress every admissible block (ACA+ at eps/10, then SVD at eps).
     const int threads = src.thread_safe() ? (nthreads > 0 ? nthreads : default_threads()) : 1;
     std::vector<LowRank> lr(approx.size());
+    if (opt->aca_device == 1) {
+        // batched ACA+ on the GPU with a rank capacity per block; recompression
+        // on host threads as blocks come back
+        std::vector<AcaBlock> blocks(static_cast<size_t>(na));
+        std::vector<int32_t> caps(static_cast<size_t>(na), std::max(1, opt->initial_rank));
+        for (int64_t b = 0; b < na; b++) {
+            const ClusterNode& a = T.nodes[approx[b].first];
+            const ClusterNode& s = S.nodes[approx[b].second];
+            blocks[b] = AcaBlock{a.start - 1, s.start - 1, a.end - a.start, s.end - s.start};
+        }
+        if (opt->predictor && na > 0) {
+            std::vector<float> X(static_cast<size_t>(na) * kRankFeatures), z(na);
+            for (int64_t b = 0; b < na; b++)
+                std::copy(h->approx_stats[b].feat, h->approx_stats[b].feat + kRankFeatures,
+                          X.begin() + b * kRankFeatures);
+            rankpred_predict(*opt->predictor, X.data(), na, z.data());
+            for (int64_t b = 0; b < na; b++) {
+                const double r = std::expm1(static_cast<double>(z[b]));
+                const double c = std::ceil((1.0 + opt->cap_margin) * std::max(r, 0.0));
+                caps[b] = static_cast<int32_t>(std::min<double>(std::max(c, 1.0), 1 << 30));
+            }
+        }
+        for (int64_t b = 0; b < na; b++)
+            h->approx_stats[b].cap = static_cast<int32_t>(
+                aca_round_cap(caps[b], std::min(blocks[b].m, blocks[b].n)));
+        h->device = opt->device;
+        gpu_aca(*h, blocks, caps, eps / 10.0, threads, opt->stats,
+                [&](int64_t b, LowRank&& f) {
+                    h->approx_stats[b].aca_rank = static_cast<int32_t>(f.k);
+                    lr[b] = svd ? svd_recompress(f, eps) : std::move(f);
+                });
+    } else
     parallel_for(static_cast<int64_t>(approx.size()), threads, [&](int64_t b) {
         const ClusterNode& a = T.nodes[approx[b].first];
         const ClusterNode& s = S.nodes[approx[b].second];
@@ -154,9 +211,11 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
             colf = [&](int64_t J, double* out) { src.block(rows.data(), mm, &cols[J], 1, out); };
         }
         LowRank f = aca_plus(mm, nn, rowf, colf, eps / 10.0, 0);  // hmatrix.jl:194-196
+        h->approx_stats[b].aca_rank = static_cast<int32_t>(f.k);
         if (svd) f = svd_recompress(f, eps);                       // :197-199
         lr[b] = std::move(f);
     });
+    for (int64_t b = 0; b < na; b++) h->approx_stats[b].rank = static_cast<int32_t>(lr[b].k);
 
     // Storage test (m+n)k < mn, else the block is stored dense after all direct
     // blocks, in approx order (hmatrix.jl:202-213).

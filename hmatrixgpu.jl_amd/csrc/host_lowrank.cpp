@@ -36,6 +36,26 @@ int64_t pick_pivot(const double* absval, int64_t n, const std::vector<char>& use
     return best;
 }
 
+}  // namespace
+
+// Sum of squares in the fixed order shared with the device ACA+ kernel: 256
+// lane partials (lane l sums x[l], x[l+256], ... in index order), then a binary
+// tree p[l] += p[l+off] for off = 128, 64, ..., 1.  (Julia's sum in aca.jl:95 is
+// blocked pairwise with SIMD reassociation, machine dependent; this order is
+// this library's definition of the stop test's rounding.)
+double sumsq_lanes(const double* x, int64_t n) {
+    double p[kSumLanes] = {};
+    for (int64_t i = 0; i < n; i++) {
+        const double sq = x[i] * x[i];
+        p[i % kSumLanes] = p[i % kSumLanes] + sq;
+    }
+    for (int off = kSumLanes / 2; off >= 1; off >>= 1)
+        for (int l = 0; l < off; l++) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+
+namespace {
+
 // Householder QR of an m x k column-major panel (m >= k), in place.
 void householder_qr(int64_t m, int64_t k, double* A, double* tau) {
     for (int64_t j = 0; j < k; j++) {
@@ -201,10 +221,8 @@ LowRank aca_plus(int64_t m, int64_t n, const RowFn& row, const RowFn& col, doubl
         for (int64_t j = 0; j < n; j++) v[j] = rrow[j] / piv;  // aca.jl:92
         std::memcpy(u, rcol.data(), sizeof(double) * m);       // aca.jl:93
         k++;
-        double su = 0.0, sv = 0.0;  // aca.jl:95, sequential summation
-        for (int64_t i = 0; i < m; i++) su += u[i] * u[i];
-        for (int64_t j = 0; j < n; j++) sv += v[j] * v[j];
-        if (std::sqrt(su * sv) < eps) break;  // aca.jl:97-99
+        const double su = sumsq_lanes(u, m), sv = sumsq_lanes(v, n);  // aca.jl:95
+        if (std::sqrt(su * sv) < eps) break;                           // aca.jl:97-99
     }
     LowRank out;
     out.m = m;

@@ -1,0 +1,469 @@
+// rankpred.hip -- MLP predictor of the ACA+ rank of an admissible block.
+//
+// No reference counterpart: aca.jl:36-106 runs ACA+ adaptively with growing
+// storage.  The predictor maps block geometry and tolerance to the rank ACA+
+// will reach, so the batched device ACA+ (aca_gpu.hip) can give every block a
+// fixed term capacity up front; blocks that outgrow it continue in a further
+// pass, which leaves the factors unchanged.
+//
+// Network: 6 -> H -> H -> 1 with ReLU, inputs
+//   log diam_s, log diam_t, log dist, log n_s, log n_t, log tol
+// normalised by the training set's mean / standard deviation, output
+// z ~ log(1 + rank).  Weights are held in BF16 for the forward pass (FP32
+// master copy for Adam), all accumulation is FP32.  Loss: mean squared error.
+//
+// Kernels:
+//   k_mlp_grad    a one-wave workgroup takes 64 samples (one per lane):
+//                 forward and backward in registers, then the weight gradients
+//                 dW2 = dA2^T H1 and dW1 = dA1^T X as LDS-tiled sums over the
+//                 workgroup's samples in sample order; one partial gradient
+//                 row per workgroup.
+//   k_mlp_adam    thread per parameter: sums the partial rows in workgroup
+//                 order (deterministic), applies Adam, refreshes the BF16 copy.
+//   k_mlp_predict thread per sample, BF16 weights expanded into LDS.
+#include "device_op.hpp"
+
+#include <cmath>
+#include <numeric>
+#include <random>
+
+namespace hmx {
+
+namespace {
+
+constexpr int kIn = 6;
+constexpr int kH = 64;
+constexpr int kGradThreads = kH;  // one wave, one sample per lane
+constexpr int kPad = kH + 1;  // LDS row stride (conflict-free per-sample rows)
+constexpr int kNParams = kIn * kH + kH + kH * kH + kH + kH + 1;
+// parameter offsets
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH;
+static_assert(oB3 + 1 == kNParams, "parameter layout");
+
+__host__ __device__ inline float bf16_round(float x) {
+    // round to nearest even on the upper 16 bits (NaN kept quiet)
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        u |= 0x00400000u;
+    else
+        u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(float, u & 0xffff0000u);
+}
+
+struct Norm {
+    float mean[kIn];
+    float inv_std[kIn];
+};
+
+// weights (BF16 values stored as FP32) -> LDS
+__device__ inline void load_weights(const float* __restrict__ wq, float* lw) {
+    for (int p = threadIdx.x; p < kNParams; p += blockDim.x) lw[p] = wq[p];
+}
+
+__device__ inline void normalise(const float* __restrict__ x, const Norm& nm, float* xn) {
+#pragma unroll
+    for (int i = 0; i < kIn; i++) xn[i] = (x[i] - nm.mean[i]) * nm.inv_std[i];
+}
+
+__global__ __launch_bounds__(kGradThreads) void k_mlp_grad(const float* __restrict__ wq,
+                                                           const float* __restrict__ X,
+                                                           const float* __restrict__ T,
+                                                           const int32_t* __restrict__ idx,
+                                                           int64_t first, int64_t count,
+                                                           float inv_batch, Norm nm,
+                                                           float* __restrict__ part,
+                                                           float* __restrict__ loss_part) {
+    __shared__ float lw[kNParams];
+    __shared__ float h1s[kGradThreads * kPad];  // H1 rows, later dA1 rows
+    __shared__ float d2s[kGradThreads * kPad];  // dA2 rows, later dz * H2 rows
+    __shared__ float xs[kGradThreads * kIn];
+    __shared__ float dzs[kGradThreads];
+    __shared__ float lred[kGradThreads];
+    load_weights(wq, lw);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kGradThreads + t;
+    const bool live = s < count;
+    float xn[kIn] = {};
+    float target = 0.f;
+    if (live) {
+        const int64_t row = idx ? idx[first + s] : first + s;
+        normalise(X + row * kIn, nm, xn);
+        target = T[row];
+    }
+    // forward
+    float h1[kH];
+    uint64_t m1 = 0, m2 = 0;
+#pragma unroll
+    for (int a = 0; a < kH; a++) {
+        float acc = lw[oB1 + a];
+#pragma unroll
+        for (int i = 0; i < kIn; i++) acc += lw[oW1 + a * kIn + i] * xn[i];
+        if (acc > 0.f) m1 |= uint64_t(1) << a;
+        h1[a] = acc > 0.f ? acc : 0.f;
+    }
+    float z = lw[oB3];
+    float h2[kH];
+#pragma unroll
+    for (int a = 0; a < kH; a++) {
+        float acc = lw[oB2 + a];
+#pragma unroll
+        for (int b = 0; b < kH; b++) acc += lw[oW2 + a * kH + b] * h1[b];
+        if (acc > 0.f) m2 |= uint64_t(1) << a;
+        h2[a] = acc > 0.f ? acc : 0.f;
+        z += lw[oW3 + a] * h2[a];
+    }
+    const float diff = live ? z - target : 0.f;
+    const float dz = 2.f * diff * inv_batch;
+    // backward: dA2 (to LDS) and dH1 = W2^T dA2
+    float d1[kH];
+#pragma unroll
+    for (int b = 0; b < kH; b++) d1[b] = 0.f;
+#pragma unroll
+    for (int a = 0; a < kH; a++) {
+        const float da2 = (m2 >> a & 1) ? dz * lw[oW3 + a] : 0.f;
+        d2s[t * kPad + a] = da2;
+#pragma unroll
+        for (int b = 0; b < kH; b++) d1[b] += lw[oW2 + a * kH + b] * da2;
+    }
+#pragma unroll
+    for (int b = 0; b < kH; b++) h1s[t * kPad + b] = h1[b];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) xs[t * kIn + i] = xn[i];
+    dzs[t] = dz;
+    lred[t] = diff * diff;
+    __syncthreads();
+    float* g = part + static_cast<int64_t>(blockIdx.x) * kNParams;
+    {  // dW2[a][:] = sum_r dA2[r][a] * H1[r][:] and db2[a], a = t, samples in order
+        float acc[kH];
+#pragma unroll
+        for (int c = 0; c < kH; c++) acc[c] = 0.f;
+        float db2 = 0.f;
+        for (int r = 0; r < kGradThreads; r++) {
+            const float d = d2s[r * kPad + t];
+            db2 += d;
+#pragma unroll
+            for (int c = 0; c < kH; c++) acc[c] += d * h1s[r * kPad + c];
+        }
+#pragma unroll
+        for (int c = 0; c < kH; c++) g[oW2 + t * kH + c] = acc[c];
+        g[oB2 + t] = db2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < kH; a++) d2s[t * kPad + a] = dz * h2[a];
+#pragma unroll
+    for (int b = 0; b < kH; b++) h1s[t * kPad + b] = (m1 >> b & 1) ? d1[b] : 0.f;
+    __syncthreads();
+    {  // dW3[t], db1[t], dW1[t][:]
+        float w3 = 0.f, db1 = 0.f;
+        float w1[kIn] = {};
+        for (int r = 0; r < kGradThreads; r++) {
+            w3 += d2s[r * kPad + t];
+            const float d = h1s[r * kPad + t];
+            db1 += d;
+#pragma unroll
+            for (int i = 0; i < kIn; i++) w1[i] += d * xs[r * kIn + i];
+        }
+        g[oW3 + t] = w3;
+        g[oB1 + t] = db1;
+#pragma unroll
+        for (int i = 0; i < kIn; i++) g[oW1 + t * kIn + i] = w1[i];
+    }
+    if (t == 0) {
+        float db3 = 0.f, l = 0.f;
+        for (int r = 0; r < kGradThreads; r++) {
+            db3 += dzs[r];
+            l += lred[r];
+        }
+        g[oB3] = db3;
+        loss_part[blockIdx.x] = l;
+    }
+}
+
+// Sum the partial gradient rows in workgroup order; optionally apply Adam.
+__global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ part, int nparts,
+                                                  float* __restrict__ grad, float* __restrict__ w,
+                                                  float* __restrict__ wq, float* __restrict__ m1,
+                                                  float* __restrict__ m2, float lr, float bc1,
+                                                  float bc2, int apply) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kNParams) return;
+    float g = 0.f;
+    for (int r = 0; r < nparts; r++) g += part[static_cast<int64_t>(r) * kNParams + p];
+    grad[p] = g;
+    if (!apply) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mm = b1 * m1[p] + (1.f - b1) * g;
+    const float vv = b2 * m2[p] + (1.f - b2) * g * g;
+    m1[p] = mm;
+    m2[p] = vv;
+    const float upd = lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    const float nw = w[p] - upd;
+    w[p] = nw;
+    wq[p] = bf16_round(nw);
+}
+
+__global__ __launch_bounds__(kGradThreads) void k_loss_sum(const float* __restrict__ loss_part,
+                                                           int n, float* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    float s = 0.f;
+    for (int i = 0; i < n; i++) s += loss_part[i];
+    out[0] += s;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ wq,
+                                                     const float* __restrict__ X, int64_t n,
+                                                     Norm nm, float* __restrict__ out) {
+    __shared__ float lw[kNParams];
+    load_weights(wq, lw);
+    __syncthreads();
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (s >= n) return;
+    float xn[kIn];
+    normalise(X + s * kIn, nm, xn);
+    float h1[kH];
+#pragma unroll
+    for (int a = 0; a < kH; a++) {
+        float acc = lw[oB1 + a];
+#pragma unroll
+        for (int i = 0; i < kIn; i++) acc += lw[oW1 + a * kIn + i] * xn[i];
+        h1[a] = acc > 0.f ? acc : 0.f;
+    }
+    float z = lw[oB3];
+    for (int a = 0; a < kH; a++) {
+        float acc = lw[oB2 + a];
+#pragma unroll
+        for (int b = 0; b < kH; b++) acc += lw[oW2 + a * kH + b] * h1[b];
+        z += lw[oW3 + a] * (acc > 0.f ? acc : 0.f);
+    }
+    out[s] = z;
+}
+
+template <typename T>
+T* dalloc(int64_t n) {
+    T* p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<int64_t>(n, 1) * sizeof(T)));
+    return p;
+}
+
+}  // namespace
+
+struct RankPredictor {
+    int device = 0;
+    int64_t step = 0;
+    Norm norm{};
+    bool norm_set = false;
+    float* w = nullptr;   // FP32 master
+    float* wq = nullptr;  // BF16 values (as FP32)
+    float* m1 = nullptr;
+    float* m2 = nullptr;
+    float* grad = nullptr;
+    ~RankPredictor() {
+        for (float* p : {w, wq, m1, m2, grad})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int64_t rankpred_nparams() { return kNParams; }
+
+RankPredictor* rankpred_create(int hidden, uint64_t seed, int device) {
+    if (hidden != kH) throw Error(HMX_ERR_UNSUPPORTED, "rank predictor: hidden width must be 64");
+    auto p = std::make_unique<RankPredictor>();
+    p->device = device;
+    DeviceGuard guard(device);
+    // He-uniform initialisation from a seeded generator, biases zero
+    std::vector<float> w(kNParams, 0.f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int fan_in, int count) {
+        const double lim = std::sqrt(6.0 / fan_in);
+        std::uniform_real_distribution<double> u(-lim, lim);
+        for (int i = 0; i < count; i++) w[off + i] = static_cast<float>(u(rng));
+    };
+    fill(oW1, kIn, kIn * kH);
+    fill(oW2, kH, kH * kH);
+    fill(oW3, kH, kH);
+    std::vector<float> wq(kNParams);
+    for (int i = 0; i < kNParams; i++) wq[i] = bf16_round(w[i]);
+    p->w = dalloc<float>(kNParams);
+    p->wq = dalloc<float>(kNParams);
+    p->m1 = dalloc<float>(kNParams);
+    p->m2 = dalloc<float>(kNParams);
+    p->grad = dalloc<float>(kNParams);
+    HIPCHECK(hipMemcpy(p->w, w.data(), kNParams * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(p->wq, wq.data(), kNParams * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(p->m1, 0, kNParams * 4));
+    HIPCHECK(hipMemset(p->m2, 0, kNParams * 4));
+    for (int i = 0; i < kIn; i++) {
+        p->norm.mean[i] = 0.f;
+        p->norm.inv_std[i] = 1.f;
+    }
+    return p.release();
+}
+
+void rankpred_destroy(RankPredictor* p) { delete p; }
+
+static void set_norm_from(RankPredictor& p, const float* X, int64_t n) {
+    for (int i = 0; i < kIn; i++) {
+        double s = 0, s2 = 0;
+        for (int64_t r = 0; r < n; r++) s += X[r * kIn + i];
+        const double mu = s / std::max<int64_t>(n, 1);
+        for (int64_t r = 0; r < n; r++) s2 += (X[r * kIn + i] - mu) * (X[r * kIn + i] - mu);
+        const double sd = std::sqrt(s2 / std::max<int64_t>(n, 1));
+        p.norm.mean[i] = static_cast<float>(mu);
+        p.norm.inv_std[i] = sd > 1e-6 ? static_cast<float>(1.0 / sd) : 1.f;
+    }
+    p.norm_set = true;
+}
+
+// One gradient evaluation over rows [first, first + count) of the (optionally
+// permuted) set; leaves the summed gradient in p.grad and, if apply, takes an
+// Adam step.  Returns the number of partial rows.
+static int grad_pass(RankPredictor& p, const float* dX, const float* dT, const int32_t* didx,
+                     int64_t first, int64_t count, float* part, float* loss_part, float* dloss,
+                     float lr, bool apply, hipStream_t st) {
+    const int nparts = static_cast<int>((count + kGradThreads - 1) / kGradThreads);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nparts), dim3(kGradThreads), 0, st, p.wq, dX, dT, didx,
+                       first, count, 1.f / static_cast<float>(count), p.norm, part, loss_part);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(kGradThreads), 0, st, loss_part, nparts, dloss);
+    float bc1 = 1.f, bc2 = 1.f;
+    if (apply) {
+        p.step++;
+        bc1 = 1.f - std::pow(0.9f, static_cast<float>(p.step));
+        bc2 = 1.f - std::pow(0.999f, static_cast<float>(p.step));
+    }
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, part, nparts,
+                       p.grad, p.w, p.wq, p.m1, p.m2, lr, bc1, bc2, apply ? 1 : 0);
+    HIPCHECK(hipGetLastError());
+    return nparts;
+}
+
+void rankpred_train(RankPredictor& p, const float* X, const float* T, int64_t n, int epochs,
+                    int batch, float lr, uint64_t seed, float* loss_hist) {
+    if (n <= 0 || epochs <= 0 || batch <= 0) throw Error(HMX_ERR_ARG, "rank predictor: empty training set");
+    DeviceGuard guard(p.device);
+    if (!p.norm_set) set_norm_from(p, X, n);
+    float* dX = dalloc<float>(n * kIn);
+    float* dT = dalloc<float>(n);
+    int32_t* didx = dalloc<int32_t>(n);
+    const int maxparts = (batch + kGradThreads - 1) / kGradThreads;
+    float* part = dalloc<float>(static_cast<int64_t>(maxparts) * kNParams);
+    float* lpart = dalloc<float>(maxparts);
+    float* dloss = dalloc<float>(1);
+    auto cleanup = [&] {
+        for (void* q : {(void*)dX, (void*)dT, (void*)didx, (void*)part, (void*)lpart, (void*)dloss})
+            (void)hipFree(q);
+    };
+    try {
+        HIPCHECK(hipMemcpy(dX, X, n * kIn * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dT, T, n * 4, hipMemcpyHostToDevice));
+        std::vector<int32_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::mt19937_64 rng(seed);
+        for (int e = 0; e < epochs; e++) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            HIPCHECK(hipMemcpy(didx, perm.data(), n * 4, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemset(dloss, 0, 4));
+            for (int64_t first = 0; first < n; first += batch) {
+                const int64_t cnt = std::min<int64_t>(batch, n - first);
+                grad_pass(p, dX, dT, didx, first, cnt, part, lpart, dloss, lr, true, nullptr);
+            }
+            if (loss_hist) {
+                float l = 0;
+                HIPCHECK(hipMemcpy(&l, dloss, 4, hipMemcpyDeviceToHost));
+                loss_hist[e] = l / static_cast<float>(n);
+            }
+        }
+        HIPCHECK(hipDeviceSynchronize());
+    } catch (...) {
+        (void)hipDeviceSynchronize();
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+void rankpred_grad(RankPredictor& p, const float* X, const float* T, int64_t n, float* grad,
+                   float* loss) {
+    if (n <= 0) throw Error(HMX_ERR_ARG, "rank predictor: empty batch");
+    DeviceGuard guard(p.device);
+    if (!p.norm_set) set_norm_from(p, X, n);
+    float* dX = dalloc<float>(n * kIn);
+    float* dT = dalloc<float>(n);
+    const int nparts = static_cast<int>((n + kGradThreads - 1) / kGradThreads);
+    float* part = dalloc<float>(static_cast<int64_t>(nparts) * kNParams);
+    float* lpart = dalloc<float>(nparts);
+    float* dloss = dalloc<float>(1);
+    auto cleanup = [&] {
+        for (void* q : {(void*)dX, (void*)dT, (void*)part, (void*)lpart, (void*)dloss}) (void)hipFree(q);
+    };
+    try {
+        HIPCHECK(hipMemcpy(dX, X, n * kIn * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dT, T, n * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemset(dloss, 0, 4));
+        grad_pass(p, dX, dT, nullptr, 0, n, part, lpart, dloss, 0.f, false, nullptr);
+        HIPCHECK(hipMemcpy(grad, p.grad, kNParams * 4, hipMemcpyDeviceToHost));
+        float l = 0;
+        HIPCHECK(hipMemcpy(&l, dloss, 4, hipMemcpyDeviceToHost));
+        if (loss) *loss = l / static_cast<float>(n);
+    } catch (...) {
+        (void)hipDeviceSynchronize();
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+void rankpred_predict(const RankPredictor& p, const float* X, int64_t n, float* z_out) {
+    if (n <= 0) return;
+    DeviceGuard guard(p.device);
+    float* dX = dalloc<float>(n * kIn);
+    float* dz = dalloc<float>(n);
+    try {
+        HIPCHECK(hipMemcpy(dX, X, n * kIn * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_mlp_predict, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                           nullptr, p.wq, dX, n, p.norm, dz);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpy(z_out, dz, n * 4, hipMemcpyDeviceToHost));
+    } catch (...) {
+        (void)hipFree(dX);
+        (void)hipFree(dz);
+        throw;
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dz);
+}
+
+void rankpred_params(RankPredictor& p, float* master, float* bf16, float* norm, bool get) {
+    DeviceGuard guard(p.device);
+    if (get) {
+        if (master) HIPCHECK(hipMemcpy(master, p.w, kNParams * 4, hipMemcpyDeviceToHost));
+        if (bf16) HIPCHECK(hipMemcpy(bf16, p.wq, kNParams * 4, hipMemcpyDeviceToHost));
+        if (norm)
+            for (int i = 0; i < kIn; i++) {
+                norm[i] = p.norm.mean[i];
+                norm[kIn + i] = p.norm.inv_std[i];
+            }
+        return;
+    }
+    if (master) {
+        std::vector<float> q(kNParams);
+        for (int i = 0; i < kNParams; i++) q[i] = bf16_round(master[i]);
+        HIPCHECK(hipMemcpy(p.w, master, kNParams * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(p.wq, q.data(), kNParams * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemset(p.m1, 0, kNParams * 4));
+        HIPCHECK(hipMemset(p.m2, 0, kNParams * 4));
+        p.step = 0;
+    }
+    if (norm) {
+        for (int i = 0; i < kIn; i++) {
+            p.norm.mean[i] = norm[i];
+            p.norm.inv_std[i] = norm[kIn + i];
+        }
+        p.norm_set = true;
+    }
+}
+
+}  // namespace hmx

@@ -7,10 +7,10 @@ hmatrix.py for the reference file:line each name mirrors.
 from ._lib import HMXError, HMX_BF16, HMX_F32, HMX_F64, LIB_PATH  # noqa: F401
 from .trees import (ACA_plus, BlockTree, BlockTreeNode, ClusterNode, ClusterTree,  # noqa: F401
                     SVD_recompress, info, merge_dense_matrices_, traverse)
-from .hmatrix import (HMatrix, HMatrixBatch, HMatrixCPU, LaplaceKernel, Log2DKernel, mul_,  # noqa: F401
-                      product, set_backend, set_groupsize)
+from .hmatrix import (HMatrix, HMatrixBatch, HMatrixCPU, LaplaceKernel, Log2DKernel,  # noqa: F401
+                      RankPredictor, mul_, product, set_backend, set_groupsize)
 
 __all__ = ["ClusterTree", "ClusterNode", "BlockTree", "BlockTreeNode", "merge_dense_matrices_",
            "traverse", "ACA_plus", "SVD_recompress", "HMatrix", "HMatrixCPU", "LaplaceKernel",
            "Log2DKernel", "mul_", "product", "info", "set_backend", "set_groupsize", "HMXError",
-           "HMatrixBatch"]
+           "HMatrixBatch", "RankPredictor"]

@@ -22,6 +22,20 @@ vp = C.c_void_p
 BLOCK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, i64p, C.c_int64, i64p, C.c_int64, f64p)
 VEC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, f64p)
 
+class BuildOptions(C.Structure):
+    _fields_ = [("aca_device", C.c_int32), ("initial_rank", C.c_int32),
+                ("predictor", C.c_void_p), ("cap_margin", C.c_double)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("blocks", "chunks", "passes", "launches",
+                                         "unconverged_first", "slot_elems", "used_elems")] + \
+               [(n, C.c_double) for n in ("aca_kernel_ms", "aca_wall_s", "build_s", "total_s")]
+
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+
 # name -> (restype, argtypes); every function returns an int status
 SIGNATURES = {
     "hmx_last_error": (C.c_char_p, []),
@@ -47,6 +61,19 @@ SIGNATURES = {
     "hmx_hmatrix_build": (C.c_int, [vp, vp, C.c_int32, f64p, f64p, BLOCK_FN, vp, C.c_double,
                                     C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(vp)]),
+    "hmx_hmatrix_build_ex": (C.c_int, [vp, vp, C.c_int32, f64p, f64p, BLOCK_FN, vp, C.c_double,
+                                       C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(BuildOptions), C.POINTER(BuildStats),
+                                       C.POINTER(vp)]),
+    "hmx_hmatrix_block_features": (C.c_int, [vp, i64p, f32p, i32p, i32p, i32p]),
+    "hmx_rankpred_create": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(vp)]),
+    "hmx_rankpred_nparams": (C.c_int, [i64p]),
+    "hmx_rankpred_train": (C.c_int, [vp, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                     C.c_uint64, f32p]),
+    "hmx_rankpred_grad": (C.c_int, [vp, f32p, f32p, C.c_int64, f32p, f32p]),
+    "hmx_rankpred_predict": (C.c_int, [vp, f32p, C.c_int64, f32p]),
+    "hmx_rankpred_params": (C.c_int, [vp, C.c_int32, f32p, f32p, f32p]),
+    "hmx_rankpred_destroy": (C.c_int, [vp]),
     "hmx_hmatrix_materialize": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "hmx_hmatrix_release": (C.c_int, [vp, C.c_int32]),
     "hmx_hmatrix_sizes": (C.c_int, [vp, i64p]),
@@ -118,3 +145,11 @@ def ptr_i64(a):
 
 def ptr_i8(a):
     return a.ctypes.data_as(i8p)
+
+
+def ptr_f32(a):
+    return a.ctypes.data_as(f32p)
+
+
+def ptr_i32(a):
+    return a.ctypes.data_as(i32p)

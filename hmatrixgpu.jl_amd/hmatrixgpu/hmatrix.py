@@ -99,7 +99,13 @@ class HMatrix:
         return super().__new__(cls)
 
     def __init__(self, K, X, Y, eta=1.5, eps=1e-5, flatten=True, index_map_using_cpu=True,
-                 svd_recompress=True, storage=("f64",), device=0, nthreads=0):
+                 svd_recompress=True, storage=("f64",), device=0, nthreads=0, aca="host",
+                 predictor=None, initial_rank=0, cap_margin=0.25):
+        """Extra keywords (no reference counterpart): storage / device of the
+        product; aca="gpu" runs the batched FP64 ACA+ on the GPU (built-in
+        kernels; identical factors), with per-block rank capacities from
+        `predictor` (a RankPredictor) or `initial_rank`.  Build statistics are in
+        self.build_stats."""
         self.K = K
         self.X, self.Y = X, Y
         self.eta, self.eps = float(eta), float(eps)
@@ -111,19 +117,26 @@ class HMatrix:
         self.device = int(device)
         h = L.vp()
         errors = []
+        if aca not in ("host", "gpu"):
+            raise ValueError("aca must be 'host' or 'gpu'")
+        opts = L.BuildOptions(1 if aca == "gpu" else 0, int(initial_rank),
+                              predictor._h if predictor is not None else None, float(cap_margin))
+        stats = L.BuildStats()
         kid = getattr(K, "kernel_id", None)
         if kid is not None:
             xt = np.ascontiguousarray(K.X.T)
             ys = np.ascontiguousarray(K.Y.T)
             cb = L.BLOCK_FN()
-            st = L.lib.hmx_hmatrix_build(X.handle, Y.handle, kid, L.ptr_f64(xt), L.ptr_f64(ys), cb,
-                                         None, self.eta, self.eps, int(bool(svd_recompress)), mask,
-                                         self.device, int(nthreads), C.byref(h))
+            st = L.lib.hmx_hmatrix_build_ex(X.handle, Y.handle, kid, L.ptr_f64(xt), L.ptr_f64(ys),
+                                            cb, None, self.eta, self.eps, int(bool(svd_recompress)),
+                                            mask, self.device, int(nthreads), C.byref(opts),
+                                            C.byref(stats), C.byref(h))
         else:
             cb = _block_callback(K, errors)
-            st = L.lib.hmx_hmatrix_build(X.handle, Y.handle, L.HMX_KERNEL_CALLBACK, None, None, cb,
-                                         None, self.eta, self.eps, int(bool(svd_recompress)), mask,
-                                         self.device, int(nthreads), C.byref(h))
+            st = L.lib.hmx_hmatrix_build_ex(X.handle, Y.handle, L.HMX_KERNEL_CALLBACK, None, None,
+                                            cb, None, self.eta, self.eps, int(bool(svd_recompress)),
+                                            mask, self.device, int(nthreads), C.byref(opts),
+                                            C.byref(stats), C.byref(h))
         if errors:
             raise errors[0]
         L.check(st)
@@ -134,6 +147,19 @@ class HMatrix:
          self._vlen) = [int(v) for v in s]
         self._layout = None
         self._scratch = {}
+        self.build_stats = {name: getattr(stats, name) for name, _ in L.BuildStats._fields_}
+
+    def block_features(self):
+        """Per admissible block (approx-list order): features (n x 6: log diam_s,
+        log diam_t, log dist, log n_s, log n_t, log tol), ACA+ rank, rank after
+        recompression and the first device ACA+ capacity (0 for a host build)."""
+        n = C.c_int64()
+        L.check(L.lib.hmx_hmatrix_block_features(self._h, C.byref(n), None, None, None, None))
+        f = np.zeros((n.value, 6), np.float32)
+        a, r, c = (np.zeros(n.value, np.int32) for _ in range(3))
+        L.check(L.lib.hmx_hmatrix_block_features(self._h, C.byref(n), L.ptr_f32(f), L.ptr_i32(a),
+                                                 L.ptr_i32(r), L.ptr_i32(c)))
+        return {"features": f, "aca_rank": a, "rank": r, "capacity": c}
 
     # ---- the reference's flattened fields (hmatrix.jl:46-57, 95-147) -------------
     def layout(self, with_dense=True):
@@ -371,6 +397,75 @@ def product(H, x, storage=None):
     else:
         y = torch.empty((x.shape[1], H.m), dtype=x.dtype, device=x.device).t()
     return mul_(y, H, x, storage=storage)
+
+
+class RankPredictor:
+    """MLP 6 -> 64 -> 64 -> 1 predicting the ACA+ rank of an admissible block
+    from HMatrix.block_features() (BF16 weights, FP32 accumulation; trained and
+    evaluated by the library's HIP kernels)."""
+
+    NPARAMS = None
+
+    def __init__(self, seed=0, device=0, hidden=64):
+        h = L.vp()
+        L.check(L.lib.hmx_rankpred_create(int(hidden), int(seed), int(device), C.byref(h)))
+        self._h = h
+        n = C.c_int64()
+        L.check(L.lib.hmx_rankpred_nparams(C.byref(n)))
+        self.nparams = n.value
+
+    @staticmethod
+    def _f32(a, cols=None):
+        a = np.ascontiguousarray(a, np.float32)
+        if cols is not None and (a.ndim != 2 or a.shape[1] != cols):
+            raise ValueError("features must be an (n, %d) array" % cols)
+        return a
+
+    def train(self, features, ranks, epochs=30, batch=4096, lr=3e-3, seed=0):
+        X, r = self._f32(features, 6), self._f32(ranks)
+        if len(r) != len(X):
+            raise ValueError("one rank per feature row")
+        hist = np.zeros(int(epochs), np.float32)
+        L.check(L.lib.hmx_rankpred_train(self._h, L.ptr_f32(X), L.ptr_f32(r), len(X), int(epochs),
+                                         int(batch), float(lr), int(seed), L.ptr_f32(hist)))
+        return hist
+
+    def grad(self, features, ranks):
+        X, r = self._f32(features, 6), self._f32(ranks)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        L.check(L.lib.hmx_rankpred_grad(self._h, L.ptr_f32(X), L.ptr_f32(r), len(X), L.ptr_f32(g),
+                                        C.byref(loss)))
+        return g, loss.value
+
+    def predict(self, features):
+        X = self._f32(features, 6)
+        out = np.zeros(len(X), np.float32)
+        L.check(L.lib.hmx_rankpred_predict(self._h, L.ptr_f32(X), len(X), L.ptr_f32(out)))
+        return out
+
+    def params(self):
+        w = np.zeros(self.nparams, np.float32)
+        q = np.zeros(self.nparams, np.float32)
+        nm = np.zeros(12, np.float32)
+        L.check(L.lib.hmx_rankpred_params(self._h, 1, L.ptr_f32(w), L.ptr_f32(q), L.ptr_f32(nm)))
+        return {"master": w, "bf16": q, "mean": nm[:6], "inv_std": nm[6:]}
+
+    def set_params(self, master=None, mean=None, inv_std=None):
+        w = None if master is None else self._f32(master)
+        nm = None
+        if mean is not None:
+            nm = np.concatenate([self._f32(mean), self._f32(inv_std)])
+        L.check(L.lib.hmx_rankpred_params(self._h, 0, None if w is None else L.ptr_f32(w), None,
+                                          None if nm is None else L.ptr_f32(nm)))
+
+    _destroy = L.lib.hmx_rankpred_destroy
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._destroy(h)
+            self._h = None
 
 
 class HMatrixBatch:

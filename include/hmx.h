@@ -119,6 +119,74 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
                       const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
                       void* ctx, double eta, double eps, int32_t svd_recompress,
                       int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out);
+/* ---- build options beyond the reference's keywords ------------------------- */
+typedef struct hmx_rankpred hmx_rankpred;
+typedef struct hmx_build_options {
+    int32_t aca_device;   /* 0: ACA+ on host threads (the default build);
+                             1: batched FP64 ACA+ on the GPU (built-in 1/r kernel) */
+    int32_t initial_rank; /* first rank capacity per block without a predictor (0: 16) */
+    const hmx_rankpred* predictor; /* NULL or a trained predictor: capacity per block
+                                      = ceil((1 + cap_margin) * predicted ACA+ rank) */
+    double cap_margin;
+} hmx_build_options;
+typedef struct hmx_build_stats {
+    int64_t blocks;            /* admissible blocks */
+    int64_t chunks;            /* memory-bounded groups of blocks of the GPU ACA+ */
+    int64_t passes;            /* capacity passes (1 = every block fit its first capacity) */
+    int64_t launches;          /* GPU ACA+ kernel launches */
+    int64_t unconverged_first; /* blocks that outgrew their first capacity */
+    int64_t slot_elems;        /* sum over passes of (m + n) * capacity (allocated terms) */
+    int64_t used_elems;        /* sum of (m + n) * ACA+ rank (filled terms) */
+    double aca_kernel_ms;      /* GPU ACA+ kernel time */
+    double aca_wall_s;         /* GPU ACA+ phase incl. copies and recompression */
+    double build_s;            /* host build (trees, ACA+, recompression, layout) */
+    double total_s;            /* build_s + device upload of the requested storage */
+} hmx_build_stats;
+/* hmx_hmatrix_build with options (NULL = defaults) and statistics (may be
+ * NULL).  The result is identical for every option: the GPU ACA+ reproduces
+ * the host ACA+ factors bit for bit, and the rank capacity only changes the
+ * schedule.  No reference counterpart beyond HMatrix(...) (hmatrix.jl:72-148,
+ * ACA_plus aca.jl:36-106), which always runs ACA+ serially on the host. */
+int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                         const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
+                         void* ctx, double eta, double eps, int32_t svd_recompress,
+                         int32_t storage_mask, int32_t device, int32_t nthreads,
+                         const hmx_build_options* options, hmx_build_stats* stats,
+                         hmx_hmatrix** out);
+/* Per admissible block in approx-list order (block.jl:127-155 traverse order):
+ * features[6*b ...] = log diam_s, log diam_t, log dist, log n_s, log n_t,
+ * log tol (diam = 2 * radius of tree.jl:86-87, dist = centre distance of
+ * block.jl:63), the ACA+ rank, the rank after recompression and the first
+ * GPU capacity (0 for a host build).  *count = number of admissible blocks;
+ * the arrays may be NULL.  No reference counterpart (training data of the rank
+ * predictor). */
+int hmx_hmatrix_block_features(const hmx_hmatrix* h, int64_t* count, float* features,
+                               int32_t* aca_rank, int32_t* rank, int32_t* capacity);
+
+/* ---- rank predictor (no reference counterpart) ------------------------------
+ * MLP 6 -> 64 -> 64 -> 1 (ReLU) from the block features above to
+ * log(1 + ACA+ rank); BF16 weights with FP32 accumulation, trained on the GPU
+ * (Adam, mean squared error).  features: n rows of 6 floats. */
+int hmx_rankpred_create(int32_t hidden /* 64 */, uint64_t seed, int32_t device,
+                        hmx_rankpred** out);
+int hmx_rankpred_nparams(int64_t* n);
+/* minibatch Adam over `epochs` shuffled passes; loss_per_epoch (may be NULL)
+ * receives the epoch's mean squared error in log(1 + rank).  The feature
+ * normalisation (mean, 1/std) is fixed from the first training set. */
+int hmx_rankpred_train(hmx_rankpred* p, const float* features, const float* ranks, int64_t n,
+                       int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                       float* loss_per_epoch);
+/* loss and its gradient w.r.t. the parameters over one batch of n samples */
+int hmx_rankpred_grad(hmx_rankpred* p, const float* features, const float* ranks, int64_t n,
+                      float* grad, float* loss);
+/* predicted ACA+ ranks (real valued) */
+int hmx_rankpred_predict(const hmx_rankpred* p, const float* features, int64_t n, float* ranks);
+/* get (get=1) or set (get=0) the FP32 master parameters, their BF16 values (get
+ * only) and the normalisation norm[0..5] = mean, norm[6..11] = 1/std.  Parameter
+ * order: W1 (64x6 row-major), b1, W2 (64x64), b2, W3 (64), b3. */
+int hmx_rankpred_params(hmx_rankpred* p, int32_t get, float* master, float* bf16, float* norm);
+int hmx_rankpred_destroy(hmx_rankpred* p);
+
 /* (Re)build the device operator of every precision in storage_mask on `device`
  * from the host-side H-matrix (the reference's kernel_array uploads,
  * hmatrix.jl:104,133-147), replacing an existing one; release frees it. */

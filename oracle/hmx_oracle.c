@@ -25,9 +25,15 @@
  *                        (test_hmatrix.jl:30-50), storage-fallback ordering
  *   orc_mul_layout       dense direct sum (orc_dense_matvec) within the H-matrix
  *                        tolerance, and numpy evaluation of the exported layout
+ *   orc_sumsq_lanes      exact on integer-valued inputs, and within n*ulp of
+ *                        a long-double sum on random inputs
+ *   orc_mlp_*            forward: constant and all-active (linear) networks vs
+ *                        numpy matrix products; gradient: central finite
+ *                        differences of orc_mlp_loss_grad's own loss
  * Parity unpinned: bitwise agreement with Julia's SIMD-reassociated sum in the
- * ACA stop test (aca.jl:95) and with LAPACK's QR/SVD bits (aca.jl:125-129);
- * only threshold-borderline rank decisions can differ.
+ * ACA stop test (aca.jl:95; here a fixed 256-lane order, orc_sumsq_lanes) and
+ * with LAPACK's QR/SVD bits (aca.jl:125-129); only threshold-borderline rank
+ * decisions can differ.
  */
 #include "hmx_oracle.h"
 
@@ -431,6 +437,22 @@ static int64_t argmax_allowed(const double* a, int64_t n, const char* used) {
     return best;
 }
 
+/* sum(x.^2) of the stop test (aca.jl:95).  Julia sums pairwise in blocks with
+ * SIMD reassociation (machine dependent); the order used here is 256 lane
+ * partials (lane l: x[l], x[l+256], ... ascending) folded by a binary tree
+ * p[l] += p[l+off], off = 128 ... 1. */
+double orc_sumsq_lanes(const double* x, int64_t n) {
+    double p[256];
+    for (int l = 0; l < 256; l++) p[l] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        double sq = x[i] * x[i];
+        p[i % 256] = p[i % 256] + sq;
+    }
+    for (int off = 128; off >= 1; off >>= 1)
+        for (int l = 0; l < off; l++) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+
 int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const int64_t* cols,
                      int64_t n, double eps, int64_t max_iter, double** Uout, double** Vout) {
     if (max_iter <= 0) max_iter = m; /* default max_iter = n_rows */
@@ -504,9 +526,7 @@ int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const 
         for (int64_t j = 0; j < n; j++) v[j] = rrow[j] / piv; /* aca.jl:92 */
         memcpy(u, rcol, (size_t)m * sizeof(double));            /* aca.jl:93 */
         k++;
-        double su = 0.0, sv = 0.0; /* aca.jl:95 (sequential order) */
-        for (int64_t i = 0; i < m; i++) su += u[i] * u[i];
-        for (int64_t j = 0; j < n; j++) sv += v[j] * v[j];
+        double su = orc_sumsq_lanes(u, m), sv = orc_sumsq_lanes(v, n); /* aca.jl:95 */
         double step = sqrt(su * sv);
         if (step < eps) break;
     }
@@ -992,4 +1012,71 @@ void orc_dense_matvec(const orc_kernel* K, const int64_t* rows, int64_t nrows, i
             for (int64_t c = 0; c < nc; c++) y[r + (c0 + c) * nrows] = acc[c];
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rank predictor MLP 6 -> 64 -> 64 -> 1 (ReLU), FP64 restatement of the     */
+/* device training kernels (no reference counterpart).  params: W1 (64x6     */
+/* row-major), b1, W2 (64x64), b2, W3 (64), b3; norm: mean[6], inv_std[6];   */
+/* loss = mean (z - t)^2.                                                     */
+#define MLP_IN 6
+#define MLP_H 64
+enum { oW1 = 0, oB1 = MLP_IN * MLP_H, oW2 = oB1 + MLP_H, oB2 = oW2 + MLP_H * MLP_H,
+       oW3 = oB2 + MLP_H, oB3 = oW3 + MLP_H, MLP_NP = oB3 + 1 };
+
+int64_t orc_mlp_nparams(void) { return MLP_NP; }
+
+static double mlp_sample(const double* P, const double* norm, const float* x, double* xn,
+                         double* a1, double* h1, double* a2, double* h2) {
+    for (int i = 0; i < MLP_IN; i++) xn[i] = ((double)x[i] - norm[i]) * norm[MLP_IN + i];
+    for (int a = 0; a < MLP_H; a++) {
+        double s = P[oB1 + a];
+        for (int i = 0; i < MLP_IN; i++) s += P[oW1 + a * MLP_IN + i] * xn[i];
+        a1[a] = s;
+        h1[a] = s > 0 ? s : 0;
+    }
+    double z = P[oB3];
+    for (int a = 0; a < MLP_H; a++) {
+        double s = P[oB2 + a];
+        for (int b = 0; b < MLP_H; b++) s += P[oW2 + a * MLP_H + b] * h1[b];
+        a2[a] = s;
+        h2[a] = s > 0 ? s : 0;
+        z += P[oW3 + a] * h2[a];
+    }
+    return z;
+}
+
+void orc_mlp_forward(const double* P, const double* norm, const float* X, int64_t n, double* z) {
+    double xn[MLP_IN], a1[MLP_H], h1[MLP_H], a2[MLP_H], h2[MLP_H];
+    for (int64_t s = 0; s < n; s++) z[s] = mlp_sample(P, norm, X + s * MLP_IN, xn, a1, h1, a2, h2);
+}
+
+void orc_mlp_loss_grad(const double* P, const double* norm, const float* X, const double* t,
+                       int64_t n, double* grad, double* loss) {
+    double xn[MLP_IN], a1[MLP_H], h1[MLP_H], a2[MLP_H], h2[MLP_H], d2[MLP_H], d1[MLP_H];
+    for (int p = 0; p < MLP_NP; p++) grad[p] = 0.0;
+    double L = 0.0;
+    for (int64_t s = 0; s < n; s++) {
+        double z = mlp_sample(P, norm, X + s * MLP_IN, xn, a1, h1, a2, h2);
+        double diff = z - t[s];
+        L += diff * diff;
+        double dz = 2.0 * diff / (double)n;
+        grad[oB3] += dz;
+        for (int a = 0; a < MLP_H; a++) {
+            grad[oW3 + a] += dz * h2[a];
+            d2[a] = a2[a] > 0 ? dz * P[oW3 + a] : 0.0;
+        }
+        for (int b = 0; b < MLP_H; b++) {
+            double s1 = 0.0;
+            for (int a = 0; a < MLP_H; a++) s1 += P[oW2 + a * MLP_H + b] * d2[a];
+            d1[b] = a1[b] > 0 ? s1 : 0.0;
+        }
+        for (int a = 0; a < MLP_H; a++) {
+            grad[oB2 + a] += d2[a];
+            for (int b = 0; b < MLP_H; b++) grad[oW2 + a * MLP_H + b] += d2[a] * h1[b];
+            grad[oB1 + a] += d1[a];
+            for (int i = 0; i < MLP_IN; i++) grad[oW1 + a * MLP_IN + i] += d1[a] * xn[i];
+        }
+    }
+    *loss = L / (double)n;
 }

@@ -76,6 +76,8 @@ void orc_btree_free(orc_btree* b);
  * buffers are malloc'd and released with orc_free. */
 int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const int64_t* cols,
                      int64_t n, double eps, int64_t max_iter, double** U, double** V);
+/* sum of squares in the stop test's fixed 256-lane order */
+double orc_sumsq_lanes(const double* x, int64_t n);
 /* SVD_recompress(U, V, eps): returns the truncated rank r; U2 m x r, V2 r x n. */
 int64_t orc_svd_recompress(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
                            double eps, double** U2, double** V2);
@@ -114,6 +116,16 @@ void orc_mul_layout_partial(const int64_t* smap, const int64_t* dense_idx, const
  * (x column-major n x nrhs, y column-major nrows x nrhs). */
 void orc_dense_matvec(const orc_kernel* K, const int64_t* rows, int64_t nrows, int64_t nrhs,
                       const double* x, double* y);
+
+/* ---- rank predictor MLP (no reference counterpart) --------------------- */
+/* 6 -> 64 -> 64 -> 1 ReLU network in FP64; params W1 (64x6 row-major), b1,
+ * W2 (64x64), b2, W3, b3; norm = mean[6], inv_std[6]; X rows of 6 floats;
+ * loss = mean over samples of (z - t)^2 and its gradient. */
+int64_t orc_mlp_nparams(void);
+void orc_mlp_forward(const double* params, const double* norm, const float* X, int64_t n,
+                     double* z);
+void orc_mlp_loss_grad(const double* params, const double* norm, const float* X,
+                       const double* t, int64_t n, double* grad, double* loss);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,11 @@ def _load():
         "orc_btree_free": (None, [vp]),
         "orc_aca_plus": (C.c_int64, [C.POINTER(Kernel), _i64p, C.c_int64, _i64p, C.c_int64,
                                      C.c_double, C.c_int64, C.POINTER(_f64p), C.POINTER(_f64p)]),
+        "orc_sumsq_lanes": (C.c_double, [_f64p, C.c_int64]),
+        "orc_mlp_nparams": (C.c_int64, []),
+        "orc_mlp_forward": (None, [_f64p, _f64p, C.POINTER(C.c_float), C.c_int64, _f64p]),
+        "orc_mlp_loss_grad": (None, [_f64p, _f64p, C.POINTER(C.c_float), _f64p, C.c_int64,
+                                     _f64p, _f64p]),
         "orc_svd_recompress": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, _f64p, _f64p,
                                            C.c_double, C.POINTER(_f64p), C.POINTER(_f64p)]),
         "orc_qr": (None, [C.c_int64, C.c_int64, _f64p, _f64p]),
@@ -208,6 +213,38 @@ def aca_plus(kernel, rows, cols, eps, max_iter=0):
     U = _take(Up, len(rows) * k).reshape(k, len(rows)).T
     V = _take(Vp, len(cols) * k).reshape(len(cols), k).T
     return U, V
+
+
+def sumsq_lanes(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib.orc_sumsq_lanes(x.ctypes.data_as(_f64p), len(x))
+
+
+def _mlp_args(params, norm, X):
+    P = np.ascontiguousarray(params, np.float64)
+    nm = np.ascontiguousarray(norm, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    assert P.shape == (lib.orc_mlp_nparams(),) and nm.shape == (12,) and X.shape[1] == 6
+    return P, nm, X
+
+
+def mlp_forward(params, norm, X):
+    P, nm, X = _mlp_args(params, norm, X)
+    z = np.zeros(len(X))
+    lib.orc_mlp_forward(P.ctypes.data_as(_f64p), nm.ctypes.data_as(_f64p),
+                        X.ctypes.data_as(C.POINTER(C.c_float)), len(X), z.ctypes.data_as(_f64p))
+    return z
+
+
+def mlp_loss_grad(params, norm, X, t):
+    P, nm, X = _mlp_args(params, norm, X)
+    t = np.ascontiguousarray(t, np.float64)
+    g = np.zeros(len(P))
+    loss = C.c_double()
+    lib.orc_mlp_loss_grad(P.ctypes.data_as(_f64p), nm.ctypes.data_as(_f64p),
+                          X.ctypes.data_as(C.POINTER(C.c_float)), t.ctypes.data_as(_f64p), len(X),
+                          g.ctypes.data_as(_f64p), C.byref(loss))
+    return g, loss.value
 
 
 def svd_recompress(U, V, eps):

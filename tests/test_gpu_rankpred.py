@@ -1,0 +1,124 @@
+"""Rank predictor on the GPU (hmx_rankpred_* through the C ABI).
+
+No reference counterpart; the checker is the oracle's FP64 restatement of the
+network (orc_mlp_*), itself pinned by finite differences and closed forms in
+test_oracle_mlp.py.  The device network computes with BF16 weights and FP32
+accumulation, so it is compared with the oracle evaluated on the same BF16
+weight values:
+  loss and gradient ... relative 2-norm error <= 1e-4
+  forward (z) ......... absolute error <= 1e-4
+Training data are (block features, ACA+ rank) pairs from the library's own
+builds.  The gain of the trained network is measured on a build it was not
+trained on, with the GPU ACA+ (identical factors with and without it).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import uniform_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(H, n, seed, tols, leaf=32):
+    pts = uniform_points(n, seed)
+    X = H.ClusterTree(pts, max_points_per_leaf=leaf)
+    F, R = [], []
+    for tol in tols:
+        hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=tol, storage=())
+        f = hm.block_features()
+        F.append(f["features"])
+        R.append(f["aca_rank"].astype(np.float32))
+    return np.concatenate(F), np.concatenate(R)
+
+
+@pytest.fixture(scope="module")
+def data(cuda):
+    import hmatrixgpu as H
+    print("[rankpred] harvesting training pairs", flush=True)
+    return _pairs(H, 4096, 1, (1e-4, 1e-6, 1e-8))
+
+
+def _oracle_inputs(P):
+    p = P.params()
+    norm = np.concatenate([p["mean"], p["inv_std"]]).astype(np.float64)
+    return p["bf16"].astype(np.float64), norm
+
+
+def test_gradient_matches_oracle(data, cuda):
+    import hmatrixgpu as H
+    F, R = data
+    P = H.RankPredictor(seed=3)
+    sel = np.random.default_rng(0).choice(len(F), 1000, replace=False)
+    # a few Adam steps first so that the check is not at the initial point only
+    P.train(F, R, epochs=1, batch=2048, lr=1e-3, seed=1)
+    g, loss = P.grad(F[sel], R[sel])
+    W, norm = _oracle_inputs(P)
+    t = np.log1p(R[sel].astype(np.float64))
+    go, lo = O.mlp_loss_grad(W, norm, F[sel], t)
+    print("[rankpred] loss gpu %.8g oracle %.8g" % (loss, lo), flush=True)
+    assert abs(loss - lo) <= 1e-4 * lo
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    for name, a, b in (("W1", 0, 384), ("W2", 448, 4544), ("W3", 4608, 4672)):
+        assert np.linalg.norm(g[a:b] - go[a:b]) <= 1e-4 * np.linalg.norm(go[a:b]) + 1e-9, name
+
+
+def test_predict_matches_oracle(data, cuda):
+    import hmatrixgpu as H
+    F, _ = data
+    P = H.RankPredictor(seed=4)
+    P.set_params(mean=F.mean(0), inv_std=1.0 / F.std(0))
+    r = P.predict(F[:3000])
+    W, norm = _oracle_inputs(P)
+    z = O.mlp_forward(W, norm, F[:3000])
+    assert np.max(np.abs(np.log1p(r.astype(np.float64)) - z)) <= 1e-4
+
+
+def test_training_is_deterministic_and_learns(data, cuda):
+    import hmatrixgpu as H
+    F, R = data
+    a, b = H.RankPredictor(seed=5), H.RankPredictor(seed=5)
+    ha = a.train(F, R, epochs=40, batch=1024, lr=3e-3, seed=2)
+    hb = b.train(F, R, epochs=40, batch=1024, lr=3e-3, seed=2)
+    assert np.array_equal(ha, hb)
+    assert np.array_equal(a.params()["master"], b.params()["master"])
+    print("[rankpred] loss first %.4f last %.5f" % (ha[0], ha[-1]), flush=True)
+    assert ha[-1] < 0.1 * ha[0]
+    mae = np.mean(np.abs(a.predict(F) - R))
+    print("[rankpred] training MAE %.3f ranks (mean rank %.2f)" % (mae, R.mean()), flush=True)
+    assert mae < 0.15 * R.mean()
+
+
+def test_predictor_gain_in_gpu_aca(data, cuda):
+    """On a build it was not trained on (N=8192, another seed): the factors are
+    identical with and without the predictor; the predictor removes most
+    continuation passes of a small fixed capacity and most of the padding of
+    a capacity large enough for one pass."""
+    import hmatrixgpu as H
+    F, R = data
+    P = H.RankPredictor(seed=6)
+    P.train(F, R, epochs=60, batch=1024, lr=3e-3, seed=3)
+    pts = uniform_points(8192, 21)
+    X = H.ClusterTree(pts, max_points_per_leaf=32)
+    K = H.LaplaceKernel(pts)
+    runs = {}
+    for name, kw in (("predictor", dict(predictor=P, cap_margin=0.25)),
+                     ("fixed16", dict(initial_rank=16)),
+                     ("fixed_max", dict(initial_rank=int(R.max()) + 8))):
+        hm = H.HMatrix(K, X, X, eta=2.0, eps=1e-6, storage=(), aca="gpu", **kw)
+        runs[name] = (hm, hm.build_stats)
+        print("[rankpred] %-9s %s" % (name, hm.build_stats), flush=True)
+    lay = [runs[k][0].layout(True) for k in runs]
+    for k in lay[0]:
+        assert np.array_equal(np.asarray(lay[0][k]), np.asarray(lay[1][k]))
+        assert np.array_equal(np.asarray(lay[0][k]), np.asarray(lay[2][k]))
+    p, f16, fmax = (runs[k][1] for k in ("predictor", "fixed16", "fixed_max"))
+    # continuation work: far fewer blocks outgrow the predicted capacity
+    assert p["unconverged_first"] <= 0.1 * f16["unconverged_first"]
+    assert p["passes"] <= f16["passes"]
+    # padding: far fewer allocated term slots than the one-pass fixed capacity
+    pad = lambda s: s["slot_elems"] - s["used_elems"]  # noqa: E731
+    assert pad(p) <= 0.5 * pad(fmax)
+    pr = runs["predictor"][0].block_features()
+    err = np.abs(P.predict(pr["features"]) - pr["aca_rank"])
+    print("[rankpred] held-out MAE %.3f ranks" % err.mean(), flush=True)

@@ -160,3 +160,24 @@ def test_recompression_of_aca_factors_matches_lapack_spectrum():
     s2 = sla.svd(U2 @ V2, compute_uv=False)
     k = U.shape[1]
     assert np.allclose(s2[:k], s_ref[:k], rtol=1e-9, atol=1e-14 * s_ref[0])
+
+
+def test_sumsq_lanes_pins():
+    # integer-valued squares below 2^53 are summed exactly in any order
+    x = np.arange(-700, 1300, dtype=np.float64)
+    assert O.sumsq_lanes(x) == float(np.sum(x.astype(np.int64) ** 2))
+    assert O.sumsq_lanes(np.zeros(0)) == 0.0
+    assert O.sumsq_lanes(np.array([3.0])) == 9.0
+    # random data: within n ulp of an extended-precision sum
+    r = np.random.default_rng(4).standard_normal(5000)
+    exact = float(np.sum(r.astype(np.longdouble) ** 2))
+    assert abs(O.sumsq_lanes(r) - exact) <= 5000 * np.finfo(float).eps * exact
+    # the fold is the documented order: lane partials then p[l] += p[l+off]
+    p = np.zeros(256)
+    for i, v in enumerate(r):
+        p[i % 256] = p[i % 256] + v * v
+    off = 128
+    while off:
+        p[:off] = p[:off] + p[off:2 * off]
+        off //= 2
+    assert O.sumsq_lanes(r) == p[0]

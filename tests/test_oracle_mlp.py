@@ -1,0 +1,81 @@
+"""Pins of the oracle's rank-predictor network (FP64 restatement of the device
+training kernels; no reference counterpart).
+
+forward  -- a constant network and an all-active (hence linear) network
+            against numpy matrix products written from the definition
+gradient -- central finite differences of the oracle's own loss
+"""
+import numpy as np
+
+import oracle_lib as O
+
+H, IN = 64, 6
+NP = IN * H + H + H * H + H + H + 1
+oW1, oB1 = 0, IN * H
+oW2 = oB1 + H
+oB2 = oW2 + H * H
+oW3 = oB2 + H
+oB3 = oW3 + H
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, IN)).astype(np.float32)
+    norm = np.concatenate([rng.standard_normal(IN) * 0.1, 1.0 + rng.random(IN)])
+    return X, norm
+
+
+def test_parameter_count():
+    assert O.lib.orc_mlp_nparams() == NP == 4673
+
+
+def test_constant_network():
+    X, norm = _data(50, 1)
+    P = np.zeros(NP)
+    P[oB3] = 2.5
+    z = O.mlp_forward(P, norm, X)
+    assert np.all(z == 2.5)
+    t = np.linspace(0, 3, 50)
+    g, loss = O.mlp_loss_grad(P, norm, X, t)
+    assert np.isclose(loss, np.mean((2.5 - t) ** 2), rtol=1e-14)
+    assert np.isclose(g[oB3], np.mean(2 * (2.5 - t)), rtol=1e-12)
+    # no hidden unit is active: every other gradient is zero
+    assert np.all(g[:oB3] == 0)
+
+
+def test_all_active_network_is_linear():
+    X, norm = _data(40, 2)
+    rng = np.random.default_rng(3)
+    W1 = rng.standard_normal((H, IN)) * 0.1
+    W2 = np.abs(rng.standard_normal((H, H))) * 0.05
+    W3 = rng.standard_normal(H)
+    b1 = np.full(H, 50.0)          # pre-activations stay positive
+    b2 = np.full(H, 1.0)
+    P = np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3, [0.7]])
+    xn = (X.astype(np.float64) - norm[:IN]) * norm[IN:]
+    a1 = xn @ W1.T + b1
+    assert np.all(a1 > 0)
+    a2 = a1 @ W2.T + b2
+    assert np.all(a2 > 0)
+    ref = a2 @ W3 + 0.7
+    assert np.allclose(O.mlp_forward(P, norm, X), ref, rtol=1e-13, atol=1e-12)
+
+
+def test_gradient_matches_finite_differences():
+    X, norm = _data(30, 4)
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal(NP) * 0.2
+    t = rng.random(30) * 3
+    g, _ = O.mlp_loss_grad(P, norm, X, t)
+    h = 1e-6
+    idx = np.concatenate([rng.choice(np.arange(oW1, oB1), 10, replace=False),
+                          rng.choice(np.arange(oB1, oW2), 5, replace=False),
+                          rng.choice(np.arange(oW2, oB2), 20, replace=False),
+                          rng.choice(np.arange(oB2, oW3), 5, replace=False),
+                          rng.choice(np.arange(oW3, oB3), 5, replace=False), [oB3]])
+    for p in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[p] += h
+        Pm[p] -= h
+        fd = (O.mlp_loss_grad(Pp, norm, X, t)[1] - O.mlp_loss_grad(Pm, norm, X, t)[1]) / (2 * h)
+        assert abs(fd - g[p]) <= 1e-6 * max(1.0, abs(g[p])), (p, fd, g[p])
