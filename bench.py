@@ -42,6 +42,14 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-gb", type=float, default=4.0,
                    help="FP64 payload bytes of the CPU-baseline sample")
+    p.add_argument("--workload", choices=("matvec", "batched", "rankpred"), default="matvec",
+                   help="matvec: the headline product (default); batched: 64 uneven "
+                        "instances x 4 RHS in one graph launch; rankpred: predictor-scheduled "
+                        "GPU ACA+ build of a 1,048,576-point jittered planar grid")
+    p.add_argument("--instances", type=int, default=64)
+    p.add_argument("--grid", type=int, default=1024, help="rankpred: grid side")
+    p.add_argument("--host-build", action="store_true",
+                   help="rankpred: also time the host-thread ACA+ build of the grid")
     p.add_argument("--profile-run", action="store_true",
                    help="workload for rocprofv3: a 4 GiB copy and a 4 GiB fill (counter "
                         "calibration), then 5 FP32 16-RHS products; prints no JSON")
@@ -174,6 +182,191 @@ def cpu_baseline(hm, info, sample_gb):
             "sample_seconds": t_sample, "host_threads_available": os.cpu_count()}
 
 
+def uneven_instances(count, seed):
+    """count point sets: N log-uniform in [2048, 65536], uniform in a box whose
+    side along one random axis is a ~ U[1, 8] (the others 1)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(2048), np.log(65536))))))
+        side = np.ones(3)
+        side[rng.integers(3)] = rng.uniform(1.0, 8.0)
+        out.append((rng.random((n, 3)) * side).T.copy())
+    return out
+
+
+def bench_batched(args):
+    """64 independent H-matrices of uneven size, 4 RHS each, FP32 storage: one
+    autotuned hipGraph launch per step vs one hmx_matvec per instance."""
+    import numpy as np
+    import torch
+
+    import hmatrixgpu as H
+
+    dev = torch.device("cuda", 0)
+    nrhs = 4
+    sets = uneven_instances(args.instances, 6)
+    log("batched: building %d instances, sizes %d..%d (total %d points), GPU ACA+"
+        % (len(sets), min(p.shape[1] for p in sets), max(p.shape[1] for p in sets),
+           sum(p.shape[1] for p in sets)))
+    t0 = time.perf_counter()
+    hms = []
+    for pts in sets:
+        X = H.ClusterTree(pts, max_points_per_leaf=args.leaf)
+        hms.append(H.HMatrix(H.LaplaceKernel(pts), X, X, eta=args.eta, eps=args.tol,
+                             storage=("f32",), aca="gpu"))
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    Xs = [torch.tensor(rng.standard_normal((h.n, nrhs)), dtype=torch.float32,
+                       device=dev).t().contiguous().t() for h in hms]
+    Ys = [torch.empty((nrhs, h.m), dtype=torch.float32, device=dev).t() for h in hms]
+    B = H.HMatrixBatch(hms, storage="f32", nrhs=nrhs)
+    tune_ms = B.autotune(Ys, Xs, iters=5)
+    cfg = B.config()
+    log("autotuned: rhs width %d, order %s (%.3f ms)" % (cfg[0], cfg[1], tune_ms))
+    tb = time_calls(torch, lambda: B.matvec(Ys, Xs), args.steps, args.warmup)
+
+    def loop():
+        for h, x, y in zip(hms, Xs, Ys):
+            H.mul_(y, h, x, storage="f32")
+    tl = time_calls(torch, loop, args.steps, args.warmup)
+    # accuracy: batched == per-instance products; spot check against FP64 direct sums
+    B.matvec(Ys, Xs)
+    torch.cuda.synchronize()
+    Yb = [y.clone() for y in Ys]
+    loop()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(Yb, Ys)) if cfg[0] == 4 else None
+    errs = []
+    for i in range(0, len(hms), max(1, len(hms) // 8)):
+        pts = sets[i]
+        rows = np.sort(np.random.default_rng(100 + i).choice(hms[i].n, 256, replace=False))
+        ref = dense_reference_rows(torch, pts, rows, Xs[i].double(), dev)
+        d = Yb[i][torch.tensor(rows, device=dev)].double() - ref
+        errs.append(float(torch.linalg.norm(d) / torch.linalg.norm(ref)))
+    alg = 0
+    for h in hms:
+        inf = h.info()
+        alg += 4 * (inf["nnz_D"] + inf["nnz_U"] + inf["nnz_V"]) + h.n * nrhs * 8
+    value = len(hms) * nrhs / tb
+    res = {
+        "metric": "batched H-matvecs/s, %d uneven instances x %d RHS, FP32" % (len(hms), nrhs),
+        "value": value, "unit": "H-matvecs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tb * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "%d H-matrices, N log-uniform [2048, 65536], boxes with one side "
+                               "a~U[1,8] (seed 6), kernel 1/r, leaf %d, eta %g, tol %g, %d RHS, "
+                               "FP32 storage, one autotuned hipGraph launch per step"
+                               % (len(hms), args.leaf, args.eta, args.tol, nrhs),
+                   "sizes": [h.n for h in hms], "parallelism": "single-gpu"},
+        "hbm_gbs": alg / tb / 1e9, "alg_bytes": alg,
+        "autotune": {"rhs_width": cfg[0], "order": cfg[1], "best_ms": tune_ms},
+        "per_instance_loop": {"ms": tl * 1e3, "H-matvecs/s": len(hms) * nrhs / tl},
+        "speedup_vs_loop": tl / tb, "bitwise_equal_to_loop": same,
+        "rel_err_vs_fp64_direct_256rows": errs, "build_s": build_s,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def jittered_grid(side, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), indexing="ij"), -1).reshape(-1, 2)
+    pts = np.concatenate([g.astype(np.float64), np.zeros((g.shape[0], 1))], 1)
+    pts += rng.uniform(-0.1, 0.1, pts.shape)
+    return pts.T.copy()
+
+
+def bench_rankpred(args):
+    """Rank predictor: trained on the GPU on (block features, ACA+ rank) pairs of
+    the uniform-cube builds (N = 4096 leaf 32, 65536 and 262144 leaf 64, tol
+    1e-6; plus 4096 and 65536 at tol 1e-4 and 1e-8), then used to give every
+    admissible block of a 1024 x 1024 jittered planar grid its GPU ACA+ capacity.
+    Compared with the GPU ACA+ at a fixed first capacity of 16 and with a fixed
+    capacity large enough for one pass.  Factors are identical in all runs."""
+    import numpy as np
+
+    import hmatrixgpu as H
+
+    F, R = [], []
+    t0 = time.perf_counter()
+    for n, leaf, seed, tols in ((4096, 32, 1, (1e-4, 1e-6, 1e-8)), (65536, 64, 2, (1e-4, 1e-6, 1e-8)),
+                                (262144, 64, 3, (1e-6,))):
+        pts = np.random.default_rng(seed).random((n, 3)).T.copy()
+        X = H.ClusterTree(pts, max_points_per_leaf=leaf)
+        for tol in tols:
+            hm = H.HMatrix(H.LaplaceKernel(pts), X, X, eta=2.0, eps=tol, storage=(), aca="gpu")
+            f = hm.block_features()
+            F.append(f["features"])
+            R.append(f["aca_rank"].astype(np.float32))
+            log("pairs: N=%d tol=%g blocks=%d mean ACA rank %.2f (%.1fs)"
+                % (n, tol, len(f["aca_rank"]), f["aca_rank"].mean(), time.perf_counter() - t0))
+            del hm
+    F, R = np.concatenate(F), np.concatenate(R)
+    harvest_s = time.perf_counter() - t0
+    P = H.RankPredictor(seed=11)
+    t0 = time.perf_counter()
+    hist = P.train(F, R, epochs=30, batch=4096, lr=3e-3, seed=12)
+    train_s = time.perf_counter() - t0
+    train_mae = float(np.mean(np.abs(P.predict(F) - R)))
+    log("trained on %d pairs in %.2fs: loss %.4f -> %.5f, MAE %.3f"
+        % (len(R), train_s, hist[0], hist[-1], train_mae))
+
+    pts = jittered_grid(args.grid, 7)
+    n = pts.shape[1]
+    X = H.ClusterTree(pts, max_points_per_leaf=args.leaf)
+    K = H.LaplaceKernel(pts)
+    runs = {}
+    kmax_seen = int(R.max())
+    for name, kw in (("predictor", dict(predictor=P, cap_margin=0.25)),
+                     ("fixed16", dict(initial_rank=16)),
+                     ("fixed_one_pass", dict(initial_rank=kmax_seen + 8))):
+        t0 = time.perf_counter()
+        hm = H.HMatrix(K, X, X, eta=2.0, eps=args.tol, storage=(), aca="gpu", **kw)
+        st = dict(hm.build_stats)
+        st["wall_s"] = time.perf_counter() - t0
+        st["padding_elems"] = st["slot_elems"] - st["used_elems"]
+        if name == "predictor":
+            f = hm.block_features()
+            pred = P.predict(f["features"])
+            st["heldout_mae"] = float(np.mean(np.abs(pred - f["aca_rank"])))
+            st["heldout_under_capacity_frac"] = float(np.mean(f["aca_rank"] <= f["capacity"]))
+            st["mean_aca_rank"] = float(f["aca_rank"].mean())
+            st["info"] = hm.info()
+        runs[name] = st
+        log("%s: %s" % (name, json.dumps(st)))
+        del hm
+    if args.host_build:
+        t0 = time.perf_counter()
+        hm = H.HMatrix(K, X, X, eta=2.0, eps=args.tol, storage=())
+        runs["host_threads"] = {"build_s": hm.build_stats["build_s"],
+                                "wall_s": time.perf_counter() - t0}
+        log("host build: %s" % json.dumps(runs["host_threads"]))
+        del hm
+    p, f16 = runs["predictor"], runs["fixed16"]
+    res = {
+        "metric": "GPU ACA+ build of N=%d planar grid with the MLP rank predictor" % n,
+        "value": p["aca_kernel_ms"], "unit": "ms (ACA+ kernels)", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "fp64 ACA+, bf16/fp32 MLP", "data": "synthetic",
+        "config": {"workload": "%dx%d jittered planar grid (spacing 1, jitter U[-0.1,0.1], seed 7), "
+                               "kernel 1/r, leaf %d, eta 2, tol %g" % (args.grid, args.grid,
+                                                                        args.leaf, args.tol),
+                   "parallelism": "single-gpu"},
+        "training": {"pairs": int(len(R)), "harvest_s": harvest_s, "train_s": train_s,
+                     "epochs": 30, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                     "train_mae": train_mae},
+        "runs": runs,
+        "gain_vs_fixed16": {"unconverged_blocks": (f16["unconverged_first"], p["unconverged_first"]),
+                            "passes": (f16["passes"], p["passes"]),
+                            "launches": (f16["launches"], p["launches"]),
+                            "aca_kernel_ms": (f16["aca_kernel_ms"], p["aca_kernel_ms"])},
+        "padding_vs_one_pass": (runs["fixed_one_pass"]["padding_elems"], p["padding_elems"]),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -184,6 +377,12 @@ def main():
         sys.exit(2)
     if args.profile_run:
         profile_run(args)
+        return
+    if args.workload == "batched":
+        bench_batched(args)
+        return
+    if args.workload == "rankpred":
+        bench_rankpred(args)
         return
     import numpy as np
     import torch

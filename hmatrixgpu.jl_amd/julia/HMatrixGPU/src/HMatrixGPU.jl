@@ -12,6 +12,7 @@ import LinearAlgebra: mul!
 import Base: *
 
 export ClusterTree, BlockTree, ACA_plus, HMatrix, info, set_groupsize, set_backend, @using_gpu
+export RankPredictor, train!, predict, block_features, HMatrixBatch, hmx_batch_mul!, BuildStats
 
 const libhmx = get(ENV, "HMX_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib", "libhmx.so"))
 
@@ -254,22 +255,77 @@ end
 Builds the H-matrix natively; with `flatten=true` the product operator is
 materialised on the GPU in `storage` precision (Float64, Float32, or :bf16).
 """
+# hmx_build_options / hmx_build_stats (include/hmx.h), C layout
+struct _BuildOptions
+    aca_device::Int32
+    initial_rank::Int32
+    predictor::Ptr{Cvoid}
+    cap_margin::Float64
+end
+mutable struct BuildStats
+    blocks::Int64; chunks::Int64; passes::Int64; launches::Int64
+    unconverged_first::Int64; slot_elems::Int64; used_elems::Int64
+    aca_kernel_ms::Float64; aca_wall_s::Float64; build_s::Float64; total_s::Float64
+    BuildStats() = new(0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0)
+end
+
+"""
+    RankPredictor(; seed=0, device=0)
+
+MLP 6 -> 64 -> 64 -> 1 predicting each admissible block's ACA+ rank from
+(log diam_s, log diam_t, log dist, log n_s, log n_t, log tol); trained and run by
+the library's HIP kernels.  `train!(p, features, ranks)` (features: n x 6).
+"""
+mutable struct RankPredictor
+    handle::Ptr{Cvoid}
+end
+function RankPredictor(; seed::Integer=0, device::Integer=0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_rankpred_create, libhmx), Cint, (Int32, UInt64, Int32, Ptr{Ptr{Cvoid}}),
+                Int32(64), UInt64(seed), Int32(device), out))
+    p = RankPredictor(out[])
+    finalizer(x -> ccall((:hmx_rankpred_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), p)
+    return p
+end
+function train!(p::RankPredictor, features::AbstractMatrix, ranks::AbstractVector;
+                epochs=30, batch=4096, lr=3f-3, seed=0)
+    F = Matrix{Float32}(permutedims(features))  # rows of 6 features, row-major
+    r = Vector{Float32}(ranks)
+    hist = Vector{Float32}(undef, epochs)
+    check(ccall((:hmx_rankpred_train, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Int64, Int32, Int32, Float32, UInt64, Ptr{Float32}),
+                p.handle, F, r, length(r), Int32(epochs), Int32(batch), Float32(lr), UInt64(seed), hist))
+    return hist
+end
+function predict(p::RankPredictor, features::AbstractMatrix)
+    F = Matrix{Float32}(permutedims(features))
+    out = Vector{Float32}(undef, size(features, 1))
+    check(ccall((:hmx_rankpred_predict, libhmx), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{Float32}),
+                p.handle, F, size(features, 1), out))
+    return out
+end
+
 function HMatrix(K::AbstractMatrix, X::ClusterTree, Y::ClusterTree; eta=1.5, eps=1e-5,
                  flatten=true, index_map_using_cpu=true, svd_recompress=true,
-                 storage=Float64, device=0)
+                 storage=Float64, device=0, aca=:host, predictor=nothing, initial_rank=0,
+                 cap_margin=0.25, stats::Union{Nothing,BuildStats}=nothing)
     st = storage === Float64 ? HMX_F64 : storage === Float32 ? HMX_F32 : HMX_BF16
     mask = flatten ? Int32(1) << st : Int32(0)
     out = Ref{Ptr{Cvoid}}(C_NULL)
     kid = _kernel_id(K)
     cb = @cfunction(_block_cb, Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}))
-    GC.@preserve K begin
+    opts = Ref(_BuildOptions(aca === :gpu ? Int32(1) : Int32(0), Int32(initial_rank),
+                             predictor === nothing ? C_NULL : predictor.handle, Float64(cap_margin)))
+    bstats = stats === nothing ? BuildStats() : stats
+    GC.@preserve K predictor begin
         xt = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.X)
         xs = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.Y)
-        check(ccall((:hmx_hmatrix_build, libhmx), Cint,
+        check(ccall((:hmx_hmatrix_build_ex, libhmx), Cint,
                     (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}, Any,
-                     Float64, Float64, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                     Float64, Float64, Int32, Int32, Int32, Int32, Ref{_BuildOptions},
+                     Ref{BuildStats}, Ptr{Ptr{Cvoid}}),
                     X.handle, Y.handle, kid, xt, xs, cb, K, Float64(eta), Float64(eps),
-                    Int32(svd_recompress), mask, Int32(device), Int32(0), out))
+                    Int32(svd_recompress), mask, Int32(device), Int32(0), opts, bstats, out))
     end
     m, n = size(K)
     T = storage === Float64 ? Float64 : Float32
@@ -348,6 +404,52 @@ function hmx_mul!(result_ptr::Ptr, H::HMatrix, x_ptr::Ptr, nrhs::Integer;
                 (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int32, Int32, Ptr{Cvoid}),
                 getfield(H, :handle), getfield(H, :storage), x_ptr, ldx, result_ptr, ldy,
                 Int32(nrhs), io, stream))
+    return nothing
+end
+
+"""
+    block_features(H) -> (features n x 6, aca_rank, rank, capacity)
+
+Training pairs of the rank predictor, one row per admissible block.
+"""
+function block_features(H::HMatrix)
+    n = Ref{Int64}(0)
+    check(ccall((:hmx_hmatrix_block_features, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Float32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+                getfield(H, :handle), n, C_NULL, C_NULL, C_NULL, C_NULL))
+    F = Matrix{Float32}(undef, 6, n[])
+    a, r, c = Vector{Int32}(undef, n[]), Vector{Int32}(undef, n[]), Vector{Int32}(undef, n[])
+    check(ccall((:hmx_hmatrix_block_features, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Float32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+                getfield(H, :handle), n, F, a, r, c))
+    return permutedims(F), a, r, c
+end
+
+"""
+    HMatrixBatch(members; nrhs=4, storage=Float32)
+
+Products of many H-matrices (each materialised in `storage`) in one captured
+hipGraph launch; `hmx_batch_mul!(B, Yptrs, Xptrs)` takes device pointers.
+"""
+mutable struct HMatrixBatch
+    handle::Ptr{Cvoid}
+    members::Vector{HMatrix}
+end
+function HMatrixBatch(members::Vector{<:HMatrix}; nrhs::Integer=4, storage=Float32)
+    st = storage === Float64 ? HMX_F64 : storage === Float32 ? HMX_F32 : HMX_BF16
+    io = storage === Float64 ? HMX_F64 : HMX_F32
+    hs = Ptr{Cvoid}[getfield(h, :handle) for h in members]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_batch_create, libhmx), Cint, (Ptr{Ptr{Cvoid}}, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                hs, Int32(length(hs)), Int32(st), Int32(nrhs), Int32(io), out))
+    B = HMatrixBatch(out[], collect(members))
+    finalizer(x -> ccall((:hmx_batch_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), B)
+    return B
+end
+function hmx_batch_mul!(B::HMatrixBatch, Y::Vector{<:Ptr}, X::Vector{<:Ptr}; stream::Ptr=C_NULL)
+    check(ccall((:hmx_batch_matvec, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Cvoid}),
+                B.handle, Ptr{Cvoid}.(X), C_NULL, Ptr{Cvoid}.(Y), C_NULL, stream))
     return nothing
 end
 
