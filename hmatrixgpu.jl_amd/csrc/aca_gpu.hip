@@ -21,7 +21,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
+#include <future>
 
 namespace hmx {
 
@@ -222,9 +224,10 @@ int64_t round_cap(int64_t c, int64_t kmax) {
 
 int64_t aca_round_cap(int64_t c, int64_t kmax) { return round_cap(c, kmax); }
 
-void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
-             const std::vector<int32_t>& caps, double eps, int nthreads, AcaStats* stats,
-             const std::function<void(int64_t, LowRank&&)>& on_done) {
+void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<int32_t> caps,
+             double eps, int nthreads, AcaStats* stats,
+             const std::function<void(int64_t, LowRank&&)>& on_done,
+             const std::vector<int64_t>& probe, const AcaRefine& refine) {
     // 1/r only: its entry (subtractions, products, sqrt, division) is correctly
     // rounded on both sides, so the device reproduces the host factors exactly;
     // the log kernel's device log() is not guaranteed to match the host's.
@@ -257,26 +260,55 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
         owned.clear();
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Host side of finished blocks: their terms are packed on the device,
+    // copied into one of two pinned staging buffers and handed to a worker that
+    // transposes V and calls on_done (recompression) on nthreads threads while
+    // the GPU proceeds with the next pass or chunk.
+    struct Staging {
+        double* host = nullptr;
+        int64_t cap = 0;
+        std::future<void> job;
+    };
+    Staging stage[2];
+    int next_stage = 0;
+    const bool verbose = std::getenv("HMX_VERBOSE") != nullptr;
+    auto wait_jobs = [&] {
+        for (auto& st : stage)
+            if (st.job.valid()) st.job.get();
+    };
+    auto free_staging = [&] {
+        for (auto& st : stage) {
+            if (st.job.valid()) st.job.wait();
+            if (st.host) (void)hipHostFree(st.host);
+            st.host = nullptr;
+        }
+    };
     try {
         HIPCHECK(hipEventCreate(&e0));
         HIPCHECK(hipEventCreate(&e1));
         const int64_t nb = static_cast<int64_t>(blocks.size());
         // most expensive first: cost ~ (m + n) * cap^2
-        std::vector<int64_t> order(nb);
-        for (int64_t b = 0; b < nb; b++) order[b] = b;
         auto cost = [&](int64_t b) {
             const double c = caps[b];
             return double(blocks[b].m + blocks[b].n) * c * c;
         };
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+        auto by_cost = [&](std::vector<int64_t>& v) {
+            std::stable_sort(v.begin(), v.end(),
+                             [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+        };
         const char* env = std::getenv("HMX_ACA_CHUNK_BYTES");
         const int64_t budget = env ? std::atoll(env) : (int64_t(8) << 30);
+        int64_t done_blocks = 0;
+        std::vector<int64_t> seen_ids;
+        std::vector<int32_t> seen_ranks;
+        // Run the listed blocks (in this order) in memory-bounded chunks.
+        auto run_list = [&](const std::vector<int64_t>& order, bool collect) {
+        const int64_t nl = static_cast<int64_t>(order.size());
         int64_t pos = 0;
-        while (pos < nb) {
+        while (pos < nl) {
             // one chunk: term slots + residual workspace within the budget
             int64_t end = pos, bytes = 0;
-            while (end < nb) {
+            while (end < nl) {
                 const AcaBlock& B = blocks[order[end]];
                 const int64_t c = round_cap(caps[order[end]], std::min(B.m, B.n));
                 const int64_t add = (B.m + B.n) * (8 * c + 9);
@@ -317,7 +349,6 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
             std::vector<int64_t> active(nc);
             for (int64_t t = 0; t < nc; t++) active[t] = t;
             double* pool = nullptr;  // previous pass's slots (continuing blocks)
-            std::vector<double> host_pool;
             int pass = 0;
             while (!active.empty()) {
                 // slots for the active blocks: U terms then V terms
@@ -370,11 +401,6 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
                 S.launches++;
                 HIPCHECK(hipMemcpy(hstate.data(), dstate, nc * sizeof(AcaState),
                                    hipMemcpyDeviceToHost));
-                host_pool.resize(static_cast<size_t>(tot));
-                HIPCHECK(hipMemcpy(host_pool.data(), npool, tot * sizeof(double),
-                                   hipMemcpyDeviceToHost));
-                // the pass has completed: release its task list, segment list and
-                // the previous pass's slots
                 release(dt);
                 release(dsegs);
                 release(pool);
@@ -382,27 +408,84 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
                 std::vector<int64_t> finished, cont;
                 for (int64_t t : active) (hstate[t].done ? finished : cont).push_back(t);
                 if (pass == 0) S.unconverged_first += static_cast<int64_t>(cont.size());
-                parallel_for(static_cast<int64_t>(finished.size()), nthreads, [&](int64_t f) {
-                    const AcaTask& T = tasks[finished[f]];
-                    const int64_t k = hstate[finished[f]].k;
-                    LowRank L;
-                    L.m = T.m;
-                    L.n = T.n;
-                    L.k = k;
-                    L.U.assign(host_pool.begin() + T.uoff, host_pool.begin() + T.uoff + k * T.m);
-                    L.V.resize(static_cast<size_t>(k * T.n));
-                    const double* src = host_pool.data() + T.voff;
-                    for (int64_t q = 0; q < k; q++)
-                        for (int64_t j = 0; j < T.n; j++) L.V[q + j * k] = src[q * T.n + j];
-                    on_done(order[pos + T.sid], std::move(L));
+
+                // pack the finished blocks' terms (U then V per block) on the device
+                struct Out {
+                    int64_t block, m, n, k, off;
+                };
+                std::vector<Out> outs;
+                std::vector<Seg> pack;
+                int64_t ptot = 0;
+                for (int64_t t : finished) {
+                    const AcaTask& T = tasks[t];
+                    const int64_t k = hstate[t].k;
+                    outs.push_back(Out{order[pos + T.sid], T.m, T.n, k, ptot});
+                    if (collect) {
+                        seen_ids.push_back(order[pos + T.sid]);
+                        seen_ranks.push_back(static_cast<int32_t>(k));
+                    }
+                    if (k > 0) {
+                        pack.push_back(Seg{T.uoff, ptot, k * T.m});
+                        pack.push_back(Seg{T.voff, ptot + k * T.m, k * T.n});
+                    }
+                    ptot += k * (T.m + T.n);
+                    S.used_elems += int64_t(T.m + T.n) * k;
+                }
+                Staging& st = stage[next_stage];
+                next_stage ^= 1;
+                if (st.job.valid()) st.job.get();  // the buffer's previous job is done
+                if (ptot > st.cap) {
+                    if (st.host) HIPCHECK(hipHostFree(st.host));
+                    st.host = nullptr;
+                    st.cap = 0;
+                    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&st.host),
+                                           ptot * sizeof(double), hipHostMallocDefault));
+                    st.cap = ptot;
+                }
+                if (!pack.empty()) {
+                    double* dpack = nullptr;
+                    HIPCHECK(hipMalloc(&dpack, ptot * sizeof(double)));
+                    owned.push_back(dpack);
+                    Seg* dps = upload(pack);
+                    owned.push_back(dps);
+                    hipLaunchKernelGGL(k_copy_segs, dim3(pack.size()), dim3(256), 0, nullptr, dps,
+                                       pool, dpack);
+                    HIPCHECK(hipGetLastError());
+                    HIPCHECK(hipMemcpy(st.host, dpack, ptot * sizeof(double),
+                                       hipMemcpyDeviceToHost));
+                    release(dps);
+                    release(dpack);
+                }
+                const double* hbuf = st.host;
+                st.job = std::async(std::launch::async, [outs = std::move(outs), hbuf, nthreads,
+                                                         &on_done] {
+                    parallel_for(static_cast<int64_t>(outs.size()), nthreads, [&](int64_t f) {
+                        const Out& o = outs[f];
+                        LowRank L;
+                        L.m = o.m;
+                        L.n = o.n;
+                        L.k = o.k;
+                        L.U.assign(hbuf + o.off, hbuf + o.off + o.k * o.m);
+                        L.V.resize(static_cast<size_t>(o.k * o.n));
+                        const double* src = hbuf + o.off + o.k * o.m;
+                        for (int64_t q = 0; q < o.k; q++)
+                            for (int64_t j = 0; j < o.n; j++) L.V[q + j * o.k] = src[q * o.n + j];
+                        on_done(o.block, std::move(L));
+                    });
                 });
-                for (int64_t t : finished) S.used_elems += int64_t(tasks[t].m + tasks[t].n) * hstate[t].k;
+                done_blocks += static_cast<int64_t>(finished.size());
                 for (int64_t t : cont) {
                     AcaTask& T = tasks[t];
                     T.kcap = static_cast<int32_t>(round_cap(2 * int64_t(T.kcap), T.kmax));
                 }
                 active.swap(cont);
                 pass++;
+                if (verbose)
+                    std::fprintf(stderr, "[hmx] GPU ACA+ chunk %lld pass %d: %lld of %lld blocks done, "
+                                         "%.1f ms kernels\n",
+                                 static_cast<long long>(S.chunks), pass,
+                                 static_cast<long long>(done_blocks), static_cast<long long>(nb),
+                                 S.kernel_ms);
             }
             S.passes = std::max<int64_t>(S.passes, pass);
             for (void* p : owned)
@@ -411,14 +494,43 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
             S.chunks++;
             pos = end;
         }
+        };
+        if (!probe.empty() && refine) {
+            // probe blocks first; their ACA+ ranks then refine the capacities of
+            // the others before those run
+            std::vector<char> in_probe(static_cast<size_t>(nb), 0);
+            std::vector<int64_t> first;
+            for (int64_t b : probe)
+                if (b >= 0 && b < nb && !in_probe[b]) {
+                    in_probe[b] = 1;
+                    first.push_back(b);
+                }
+            by_cost(first);
+            run_list(first, true);
+            S.probe_blocks = static_cast<int64_t>(first.size());
+            refine(seen_ids, seen_ranks, caps);
+            std::vector<int64_t> rest;
+            for (int64_t b = 0; b < nb; b++)
+                if (!in_probe[b]) rest.push_back(b);
+            by_cost(rest);
+            run_list(rest, false);
+        } else {
+            std::vector<int64_t> all(nb);
+            for (int64_t b = 0; b < nb; b++) all[b] = b;
+            by_cost(all);
+            run_list(all, false);
+        }
+        wait_jobs();
         S.blocks = nb;
     } catch (...) {
         (void)hipDeviceSynchronize();
+        free_staging();
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         free_all();
         throw;
     }
+    free_staging();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     free_all();

@@ -280,6 +280,10 @@ int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t
             if (options->initial_rank > 0) opt.initial_rank = options->initial_rank;
             opt.predictor = options->predictor ? options->predictor->p.get() : nullptr;
             opt.cap_margin = options->cap_margin;
+            need(options->refine_samples >= 0 && options->refine_epochs >= 0,
+                 "hmx_hmatrix_build_ex: negative refine_samples or refine_epochs");
+            opt.refine_samples = options->refine_samples;
+            if (options->refine_epochs > 0) opt.refine_epochs = options->refine_epochs;
             if (opt.aca_device == 1 && kernel_id != HMX_KERNEL_LAPLACE)
                 throw hmx::Error(HMX_ERR_UNSUPPORTED, "GPU ACA+ supports the built-in 1/r kernel");
         }
@@ -320,6 +324,7 @@ int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t
             stats->passes = ast.passes;
             stats->launches = ast.launches;
             stats->unconverged_first = ast.unconverged_first;
+            stats->probe_blocks = ast.probe_blocks;
             stats->slot_elems = ast.slot_elems;
             stats->used_elems = ast.used_elems;
             stats->aca_kernel_ms = ast.kernel_ms;

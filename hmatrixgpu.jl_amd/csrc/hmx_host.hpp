@@ -177,6 +177,7 @@ struct AcaStats {
     int64_t passes = 0;             // most capacity passes any chunk needed
     int64_t launches = 0;           // ACA+ kernel launches
     int64_t unconverged_first = 0;  // blocks that outgrew their first capacity
+    int64_t probe_blocks = 0;       // blocks run before the capacities were refined
     int64_t slot_elems = 0;         // sum over passes of (m + n) * capacity
     int64_t used_elems = 0;         // sum of (m + n) * final rank
     double kernel_ms = 0.0;         // ACA+ kernel time (HIP events)
@@ -185,9 +186,14 @@ struct AcaStats {
 // ACA+ (eps is the ACA tolerance) of every block with the given initial rank
 // capacities; on_done(b, factors) is called from host worker threads as blocks
 // finish.  Built-in scalar kernels only.
-void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks,
-             const std::vector<int32_t>& caps, double eps, int nthreads, AcaStats* stats,
-             const std::function<void(int64_t, LowRank&&)>& on_done);
+// Optional probe phase: the blocks listed in `probe` run first, then
+// refine(ids, aca_ranks, caps) may rewrite the capacities of the other blocks.
+using AcaRefine = std::function<void(const std::vector<int64_t>&, const std::vector<int32_t>&,
+                                     std::vector<int32_t>&)>;
+void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<int32_t> caps,
+             double eps, int nthreads, AcaStats* stats,
+             const std::function<void(int64_t, LowRank&&)>& on_done,
+             const std::vector<int64_t>& probe = {}, const AcaRefine& refine = nullptr);
 // capacity rounding of a pass (multiple of 4, at least 4, at most kmax)
 int64_t aca_round_cap(int64_t c, int64_t kmax);
 
@@ -197,6 +203,7 @@ int64_t aca_round_cap(int64_t c, int64_t kmax);
 struct RankPredictor;
 RankPredictor* rankpred_create(int hidden, uint64_t seed, int device);
 void rankpred_destroy(RankPredictor* p);
+RankPredictor* rankpred_clone(const RankPredictor& p);
 int64_t rankpred_nparams();
 void rankpred_train(RankPredictor& p, const float* X, const float* Z, int64_t n, int epochs,
                     int batch, float lr, uint64_t seed, float* loss_hist);
@@ -213,6 +220,8 @@ struct BuildOptions {
     int initial_rank = 16;                    // first capacity without a predictor
     const RankPredictor* predictor = nullptr;  // capacity from the predicted rank
     double cap_margin = 0.25;                 // capacity = (1 + margin) * predicted rank
+    int refine_samples = 0;                   // probe blocks that fine-tune a copy of the
+    int refine_epochs = 200;                  // predictor on this build's own ACA+ ranks
     AcaStats* stats = nullptr;
 };
 

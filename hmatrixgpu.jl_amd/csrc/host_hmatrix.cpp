@@ -169,27 +169,68 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
             const ClusterNode& s = S.nodes[approx[b].second];
             blocks[b] = AcaBlock{a.start - 1, s.start - 1, a.end - a.start, s.end - s.start};
         }
-        if (opt->predictor && na > 0) {
-            std::vector<float> X(static_cast<size_t>(na) * kRankFeatures), z(na);
-            for (int64_t b = 0; b < na; b++)
-                std::copy(h->approx_stats[b].feat, h->approx_stats[b].feat + kRankFeatures,
-                          X.begin() + b * kRankFeatures);
-            rankpred_predict(*opt->predictor, X.data(), na, z.data());
+        std::vector<float> X(static_cast<size_t>(na) * kRankFeatures);
+        for (int64_t b = 0; b < na; b++)
+            std::copy(h->approx_stats[b].feat, h->approx_stats[b].feat + kRankFeatures,
+                      X.begin() + b * kRankFeatures);
+        // capacity = ceil((1 + margin) * predicted rank)
+        auto predict_caps = [&](const RankPredictor& p, std::vector<int32_t>& out,
+                                const std::vector<char>* skip) {
+            std::vector<float> z(static_cast<size_t>(na));
+            rankpred_predict(p, X.data(), na, z.data());
             for (int64_t b = 0; b < na; b++) {
+                if (skip && (*skip)[b]) continue;
                 const double r = std::expm1(static_cast<double>(z[b]));
                 const double c = std::ceil((1.0 + opt->cap_margin) * std::max(r, 0.0));
-                caps[b] = static_cast<int32_t>(std::min<double>(std::max(c, 1.0), 1 << 30));
+                out[b] = static_cast<int32_t>(std::min<double>(std::max(c, 1.0), 1 << 30));
             }
+        };
+        if (opt->predictor && na > 0) predict_caps(*opt->predictor, caps, nullptr);
+        auto record_caps = [&](const std::vector<char>* skip) {
+            for (int64_t b = 0; b < na; b++)
+                if (!skip || !(*skip)[b])
+                    h->approx_stats[b].cap = static_cast<int32_t>(
+                        aca_round_cap(caps[b], std::min(blocks[b].m, blocks[b].n)));
+        };
+        record_caps(nullptr);
+        // Probe phase: an evenly spaced sample of the admissible blocks (DFS
+        // order, so every level and region is represented) runs first; a copy
+        // of the predictor is fine-tuned on their ACA+ ranks and predicts the
+        // capacities of the remaining blocks.
+        std::vector<int64_t> probe;
+        std::vector<char> in_probe;
+        AcaRefine refine;
+        if (opt->predictor && opt->refine_samples > 0 && na >= 4 * int64_t(opt->refine_samples)) {
+            const int64_t step = na / opt->refine_samples;
+            in_probe.assign(static_cast<size_t>(na), 0);
+            for (int64_t b = 0; b < na; b += step) {
+                probe.push_back(b);
+                in_probe[b] = 1;
+            }
+            refine = [&](const std::vector<int64_t>& ids, const std::vector<int32_t>& ranks,
+                         std::vector<int32_t>& c) {
+                std::unique_ptr<RankPredictor, void (*)(RankPredictor*)> q(
+                    rankpred_clone(*opt->predictor), rankpred_destroy);
+                std::vector<float> Xs(ids.size() * kRankFeatures), zs(ids.size());
+                for (size_t i = 0; i < ids.size(); i++) {
+                    std::copy(X.begin() + ids[i] * kRankFeatures,
+                              X.begin() + (ids[i] + 1) * kRankFeatures, Xs.begin() + i * kRankFeatures);
+                    zs[i] = static_cast<float>(std::log1p(static_cast<double>(ranks[i])));
+                }
+                rankpred_train(*q, Xs.data(), zs.data(), static_cast<int64_t>(ids.size()),
+                               opt->refine_epochs, 256, 3e-3f, 0x5eedULL, nullptr);
+                predict_caps(*q, c, &in_probe);
+                caps = c;
+                record_caps(&in_probe);
+            };
         }
-        for (int64_t b = 0; b < na; b++)
-            h->approx_stats[b].cap = static_cast<int32_t>(
-                aca_round_cap(caps[b], std::min(blocks[b].m, blocks[b].n)));
         h->device = opt->device;
         gpu_aca(*h, blocks, caps, eps / 10.0, threads, opt->stats,
                 [&](int64_t b, LowRank&& f) {
                     h->approx_stats[b].aca_rank = static_cast<int32_t>(f.k);
                     lr[b] = svd ? svd_recompress(f, eps) : std::move(f);
-                });
+                },
+                probe, refine);
     } else
     parallel_for(static_cast<int64_t>(approx.size()), threads, [&](int64_t b) {
         const ClusterNode& a = T.nodes[approx[b].first];

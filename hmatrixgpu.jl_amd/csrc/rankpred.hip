@@ -304,6 +304,22 @@ RankPredictor* rankpred_create(int hidden, uint64_t seed, int device) {
 
 void rankpred_destroy(RankPredictor* p) { delete p; }
 
+// Copy of the weights and normalisation with a fresh optimiser state (Adam
+// moments zero, step 0), for fine-tuning on new data.
+RankPredictor* rankpred_clone(const RankPredictor& src) {
+    DeviceGuard guard(src.device);
+    auto p = std::make_unique<RankPredictor>();
+    p->device = src.device;
+    p->norm = src.norm;
+    p->norm_set = src.norm_set;
+    for (float** dst : {&p->w, &p->wq, &p->m1, &p->m2, &p->grad}) *dst = dalloc<float>(kNParams);
+    HIPCHECK(hipMemcpy(p->w, src.w, kNParams * 4, hipMemcpyDeviceToDevice));
+    HIPCHECK(hipMemcpy(p->wq, src.wq, kNParams * 4, hipMemcpyDeviceToDevice));
+    HIPCHECK(hipMemset(p->m1, 0, kNParams * 4));
+    HIPCHECK(hipMemset(p->m2, 0, kNParams * 4));
+    return p.release();
+}
+
 static void set_norm_from(RankPredictor& p, const float* X, int64_t n) {
     for (int i = 0; i < kIn; i++) {
         double s = 0, s2 = 0;

@@ -24,12 +24,14 @@ VEC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, f64p)
 
 class BuildOptions(C.Structure):
     _fields_ = [("aca_device", C.c_int32), ("initial_rank", C.c_int32),
-                ("predictor", C.c_void_p), ("cap_margin", C.c_double)]
+                ("predictor", C.c_void_p), ("cap_margin", C.c_double),
+                ("refine_samples", C.c_int32), ("refine_epochs", C.c_int32)]
 
 
 class BuildStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("blocks", "chunks", "passes", "launches",
-                                         "unconverged_first", "slot_elems", "used_elems")] + \
+                                         "unconverged_first", "probe_blocks", "slot_elems",
+                                         "used_elems")] + \
                [(n, C.c_double) for n in ("aca_kernel_ms", "aca_wall_s", "build_s", "total_s")]
 
 

@@ -100,12 +100,14 @@ class HMatrix:
 
     def __init__(self, K, X, Y, eta=1.5, eps=1e-5, flatten=True, index_map_using_cpu=True,
                  svd_recompress=True, storage=("f64",), device=0, nthreads=0, aca="host",
-                 predictor=None, initial_rank=0, cap_margin=0.25):
+                 predictor=None, initial_rank=0, cap_margin=0.25, refine_samples=0,
+                 refine_epochs=0):
         """Extra keywords (no reference counterpart): storage / device of the
         product; aca="gpu" runs the batched FP64 ACA+ on the GPU (built-in
         kernels; identical factors), with per-block rank capacities from
-        `predictor` (a RankPredictor) or `initial_rank`.  Build statistics are in
-        self.build_stats."""
+        `predictor` (a RankPredictor) or `initial_rank`; refine_samples > 0 first
+        runs that many evenly spaced blocks and fine-tunes a copy of the predictor
+        on their ranks.  Build statistics are in self.build_stats."""
         self.K = K
         self.X, self.Y = X, Y
         self.eta, self.eps = float(eta), float(eps)
@@ -120,7 +122,8 @@ class HMatrix:
         if aca not in ("host", "gpu"):
             raise ValueError("aca must be 'host' or 'gpu'")
         opts = L.BuildOptions(1 if aca == "gpu" else 0, int(initial_rank),
-                              predictor._h if predictor is not None else None, float(cap_margin))
+                              predictor._h if predictor is not None else None, float(cap_margin),
+                              int(refine_samples), int(refine_epochs))
         stats = L.BuildStats()
         kid = getattr(K, "kernel_id", None)
         if kid is not None:

@@ -261,12 +261,14 @@ struct _BuildOptions
     initial_rank::Int32
     predictor::Ptr{Cvoid}
     cap_margin::Float64
+    refine_samples::Int32
+    refine_epochs::Int32
 end
 mutable struct BuildStats
     blocks::Int64; chunks::Int64; passes::Int64; launches::Int64
-    unconverged_first::Int64; slot_elems::Int64; used_elems::Int64
+    unconverged_first::Int64; probe_blocks::Int64; slot_elems::Int64; used_elems::Int64
     aca_kernel_ms::Float64; aca_wall_s::Float64; build_s::Float64; total_s::Float64
-    BuildStats() = new(0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0)
+    BuildStats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0)
 end
 
 """
@@ -308,14 +310,16 @@ end
 function HMatrix(K::AbstractMatrix, X::ClusterTree, Y::ClusterTree; eta=1.5, eps=1e-5,
                  flatten=true, index_map_using_cpu=true, svd_recompress=true,
                  storage=Float64, device=0, aca=:host, predictor=nothing, initial_rank=0,
-                 cap_margin=0.25, stats::Union{Nothing,BuildStats}=nothing)
+                 cap_margin=0.25, refine_samples=0, refine_epochs=0,
+                 stats::Union{Nothing,BuildStats}=nothing)
     st = storage === Float64 ? HMX_F64 : storage === Float32 ? HMX_F32 : HMX_BF16
     mask = flatten ? Int32(1) << st : Int32(0)
     out = Ref{Ptr{Cvoid}}(C_NULL)
     kid = _kernel_id(K)
     cb = @cfunction(_block_cb, Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}))
     opts = Ref(_BuildOptions(aca === :gpu ? Int32(1) : Int32(0), Int32(initial_rank),
-                             predictor === nothing ? C_NULL : predictor.handle, Float64(cap_margin)))
+                             predictor === nothing ? C_NULL : predictor.handle, Float64(cap_margin),
+                             Int32(refine_samples), Int32(refine_epochs)))
     bstats = stats === nothing ? BuildStats() : stats
     GC.@preserve K predictor begin
         xt = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.X)

@@ -128,6 +128,11 @@ typedef struct hmx_build_options {
     const hmx_rankpred* predictor; /* NULL or a trained predictor: capacity per block
                                       = ceil((1 + cap_margin) * predicted ACA+ rank) */
     double cap_margin;
+    int32_t refine_samples; /* > 0 (with a predictor): ACA+ first runs on about this many
+                               evenly spaced blocks, a copy of the predictor is fine-tuned
+                               on their ranks (refine_epochs, on the GPU) and predicts the
+                               capacities of the rest; the caller's predictor is unchanged */
+    int32_t refine_epochs;  /* 0: 200 (minibatches of 256, Adam from a fresh state) */
 } hmx_build_options;
 typedef struct hmx_build_stats {
     int64_t blocks;            /* admissible blocks */
@@ -135,6 +140,7 @@ typedef struct hmx_build_stats {
     int64_t passes;            /* capacity passes (1 = every block fit its first capacity) */
     int64_t launches;          /* GPU ACA+ kernel launches */
     int64_t unconverged_first; /* blocks that outgrew their first capacity */
+    int64_t probe_blocks;      /* blocks run before the capacities were refined */
     int64_t slot_elems;        /* sum over passes of (m + n) * capacity (allocated terms) */
     int64_t used_elems;        /* sum of (m + n) * ACA+ rank (filled terms) */
     double aca_kernel_ms;      /* GPU ACA+ kernel time */

@@ -126,3 +126,35 @@ def test_predictor_gain_in_gpu_aca(data, cuda):
     pr = runs["predictor"][0].block_features()
     err = np.abs(P.predict(pr["features"]) - pr["aca_rank"])
     print("[rankpred] held-out MAE %.3f ranks" % err.mean(), flush=True)
+
+
+def test_probe_refinement_under_distribution_shift(data, cuda):
+    """A predictor trained on unit-cube blocks meets a planar grid (points one
+    unit apart): its geometric inputs fall outside the training range.  The
+    probe phase fine-tunes a copy on the grid's own ACA+ ranks; factors stay
+    identical and the padding drops against the unrefined predictor."""
+    import hmatrixgpu as H
+    F, R = data
+    P = H.RankPredictor(seed=7)
+    P.train(F, R, epochs=40, batch=1024, lr=3e-3, seed=4)
+    side = 160
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), indexing="ij"), -1).reshape(-1, 2)
+    rng = np.random.default_rng(7)
+    pts = (np.concatenate([g, np.zeros((len(g), 1))], 1) + rng.uniform(-0.1, 0.1, (len(g), 3))).T.copy()
+    X = H.ClusterTree(pts, max_points_per_leaf=32)
+    K = H.LaplaceKernel(pts)
+    host = H.HMatrix(K, X, X, eta=2.0, eps=1e-6, storage=())
+    plain = H.HMatrix(K, X, X, eta=2.0, eps=1e-6, storage=(), aca="gpu", predictor=P)
+    refined = H.HMatrix(K, X, X, eta=2.0, eps=1e-6, storage=(), aca="gpu", predictor=P,
+                        refine_samples=1024)
+    for hm in (plain, refined):
+        lay, ref = hm.layout(True), host.layout(True)
+        for k in ref:
+            assert np.array_equal(np.asarray(lay[k]), np.asarray(ref[k])), k
+    sp, sr = plain.build_stats, refined.build_stats
+    print("[rankpred] plain %s\n[rankpred] refined %s" % (sp, sr), flush=True)
+    assert 1024 <= sr["probe_blocks"] <= 1100
+    pad = lambda s: s["slot_elems"] - s["used_elems"]  # noqa: E731
+    assert pad(sr) < 0.5 * pad(sp)
+    f = refined.block_features()
+    assert np.mean(f["aca_rank"] <= f["capacity"]) >= 0.9
