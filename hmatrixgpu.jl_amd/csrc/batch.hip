@@ -47,7 +47,7 @@ struct Batch {
     // device work lists and descriptor table
     FarItem* dA = nullptr;
     ReduceGroup* dG = nullptr;
-    Item* dB = nullptr;
+    PairItem* dB = nullptr;
     StageArrays arrays;
     InstDesc* dInst = nullptr;
     std::vector<InstDesc> hInst;  // host image of the table
@@ -102,7 +102,7 @@ void build_lists(Batch& b) {
     drop_graph(b);
     std::vector<FarItem> A;
     std::vector<ReduceGroup> G;
-    std::vector<Item> B;
+    std::vector<PairItem> B;
     int64_t gmax = 0;
     b.hInst.assign(b.ops.size(), InstDesc{});
     for (size_t i = 0; i < b.ops.size(); i++) {
@@ -115,7 +115,7 @@ void build_lists(Batch& b) {
             g.inst = static_cast<int32_t>(i);
             G.push_back(g);
         }
-        for (Item it : op.hItemsB) {
+        for (PairItem it : op.hItemsB) {
             it.inst = static_cast<int32_t>(i);
             B.push_back(it);
         }

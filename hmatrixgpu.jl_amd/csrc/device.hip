@@ -56,26 +56,33 @@ constexpr int kXcds = 8;
 
 }  // namespace
 
-void order_leaf_items(std::vector<Item>& items, const char* env) {
+// Work of a leaf item in panel elements per wave plus a fixed cost.
+int64_t leaf_item_work(const PairItem& it) {
+    const int64_t a = int64_t(it.sh_kt) * (it.rows[0] + it.rows[1]) +
+                      int64_t(it.pr_kt[0]) * it.rows[0] + int64_t(it.pr_kt[1]) * it.rows[1];
+    return a * 64 + 4096;
+}
+
+void order_leaf_items(std::vector<PairItem>& items, const char* env) {
     if (items.size() < 2 * kXcds) return;
-    auto work = [](const Item& it) { return int64_t(it.ktiles) * it.rows * 64 + 4096; };
+    auto work = leaf_item_work;
     if (!env || std::string(env) == "lpt") {
         std::stable_sort(items.begin(), items.end(),
-                         [&](const Item& a, const Item& b) { return work(a) > work(b); });
+                         [&](const PairItem& a, const PairItem& b) { return work(a) > work(b); });
         return;
     }
     if (env && std::string(env) == "tree") return;
     int64_t total = 0;
-    for (const Item& it : items) total += work(it);
-    std::vector<std::vector<Item>> chunk(kXcds);
+    for (const PairItem& it : items) total += work(it);
+    std::vector<std::vector<PairItem>> chunk(kXcds);
     int64_t acc = 0;
-    for (const Item& it : items) {
+    for (const PairItem& it : items) {
         const int c = static_cast<int>(std::min<int64_t>(kXcds - 1, acc * kXcds / total));
         chunk[c].push_back(it);
         acc += work(it);
     }
     std::vector<size_t> pos(kXcds, 0);
-    std::vector<Item> out;
+    std::vector<PairItem> out;
     out.reserve(items.size());
     for (size_t p = 0; out.size() < items.size(); p++) {
         int c = static_cast<int>(p % kXcds);
@@ -126,7 +133,7 @@ int64_t device_panel_bytes(const DeviceOp* op) {
 int64_t device_index_bytes(const DeviceOp* op) {
     if (!op) return 0;
     return op->ncol * 4 + op->nA * static_cast<int64_t>(sizeof(FarItem)) +
-           op->nB * static_cast<int64_t>(sizeof(Item)) +
+           op->nB * static_cast<int64_t>(sizeof(PairItem)) +
            op->nG * static_cast<int64_t>(sizeof(ReduceGroup)) + (op->m + op->n) * 4;
 }
 
@@ -235,48 +242,148 @@ void device_build(HMatrix& h, int storage) {
         for (int64_t l = a; l < b; l++) lr_of[l].push_back(static_cast<int32_t>(d));
     }
     const int64_t zero_row = h.n + h.R;
-    std::vector<Item> itemsB;
+    const bool builtin = h.src.kernel_id != HMX_KERNEL_CALLBACK;
+    // Row groups (<= 64 rows) in target order; consecutive row groups are paired
+    // when they share columns (blocks covering both), choosing the pairing that
+    // maximises the shared columns (dynamic programme over the sequence).
+    struct Unit {
+        int64_t r0;
+        int32_t rows;
+        int64_t leaf;
+    };
+    std::vector<Unit> units;
+    for (int64_t l = 0; l < nleaf; l++)
+        for (int64_t r0 = ls[l]; r0 < ls[l + 1]; r0 += 64)
+            units.push_back({r0, static_cast<int32_t>(std::min<int64_t>(64, ls[l + 1] - r0)), l});
+    const int64_t nu = static_cast<int64_t>(units.size());
+    auto intersect = [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) {
+        std::vector<int32_t> out;
+        std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+        return out;
+    };
+    auto minus = [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) {
+        std::vector<int32_t> out;
+        std::set_difference(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+        return out;
+    };
+    auto ncols_of = [&](const std::vector<int32_t>& dl, const std::vector<int32_t>& ll) {
+        int64_t c = 0;
+        for (int32_t d : dl) c += h.dense[d].c1 - h.dense[d].c0 + 1;
+        for (int32_t b : ll) c += h.lowrank[b].k;
+        return c;
+    };
+    std::vector<int64_t> gain(static_cast<size_t>(std::max<int64_t>(nu, 1)), 0);
+    for (int64_t u = 0; u + 1 < nu; u++) {
+        const int64_t la = units[u].leaf, lb = units[u + 1].leaf;
+        const int64_t c = ncols_of(intersect(dense_of[la], dense_of[lb]),
+                                   intersect(lr_of[la], lr_of[lb]));
+        gain[u] = c >= 2 * TK ? c : 0;  // too few shared columns: not worth a segment
+    }
+    std::vector<int64_t> best(static_cast<size_t>(nu + 1), 0);
+    std::vector<char> take(static_cast<size_t>(nu + 1), 0);
+    for (int64_t i = 2; i <= nu; i++) {
+        best[i] = best[i - 1];
+        if (gain[i - 2] > 0 && best[i - 2] + gain[i - 2] > best[i]) {
+            best[i] = best[i - 2] + gain[i - 2];
+            take[i] = 1;
+        }
+    }
+    std::vector<std::pair<int64_t, int64_t>> groups_b;  // (first unit, second unit or -1)
+    for (int64_t i = nu; i > 0;) {
+        if (i >= 2 && take[i]) {
+            groups_b.push_back({i - 2, i - 1});
+            i -= 2;
+        } else {
+            groups_b.push_back({i - 1, -1});
+            i -= 1;
+        }
+    }
+    std::reverse(groups_b.begin(), groups_b.end());
+
+    std::vector<PairItem> itemsB;
+    std::vector<Item> gensB;  // one generation item per panel (col_off into cdesc)
     std::vector<int32_t> colidx;
     std::vector<ColDesc> cdesc;
-    const bool builtin = h.src.kernel_id != HMX_KERNEL_CALLBACK;
-    for (int64_t l = 0; l < nleaf; l++) {
-        for (int64_t r0 = ls[l]; r0 < ls[l + 1]; r0 += 64) {
-            const int32_t rows = static_cast<int32_t>(std::min<int64_t>(64, ls[l + 1] - r0));
-            const int64_t col_off = static_cast<int64_t>(colidx.size());
-            for (int32_t d : dense_of[l]) {
-                const DenseBlock& D = h.dense[d];
-                const int64_t nn = D.c1 - D.c0 + 1;
-                for (int64_t j = 0; j < nn; j++) {
-                    colidx.push_back(static_cast<int32_t>(D.c0 - 1 + j));
-                    if (builtin)
-                        cdesc.push_back({h.smap[D.c0 - 1 + j] - 1, 0, 0});
-                    else
-                        cdesc.push_back({D.off + (r0 - (D.r0 - 1)) * nn + j, static_cast<int32_t>(nn), 2});
-                }
-            }
-            for (int32_t b : lr_of[l]) {
-                const LowRankBlock& L = h.lowrank[b];
-                for (int64_t j = 0; j < L.k; j++) {
-                    colidx.push_back(static_cast<int32_t>(h.n + trow_of_block[b] + j));
-                    cdesc.push_back({L.uoff + (r0 - (L.r0 - 1)) * L.k + j, static_cast<int32_t>(L.k), 1});
-                }
-            }
-            while ((colidx.size() - col_off) % TK) {
-                colidx.push_back(static_cast<int32_t>(zero_row));
-                cdesc.push_back({0, 0, 3});
-            }
-            const int64_t K = static_cast<int64_t>(colidx.size()) - col_off;
-            Item it{};
-            it.data_off = data_off;
-            it.col_off = col_off;
-            it.out_off = r0;
-            it.rows = rows;
-            it.ktiles = static_cast<int32_t>(K / TK);
-            it.ncols = static_cast<int32_t>(K);
-            it.inst = 0;
-            itemsB.push_back(it);
-            data_off += K * rows;
+    // append the columns of the listed blocks for row group starting at r0
+    auto add_colidx = [&](const std::vector<int32_t>& dl, const std::vector<int32_t>& ll) {
+        const int64_t c0 = static_cast<int64_t>(colidx.size());
+        for (int32_t d : dl) {
+            const DenseBlock& D = h.dense[d];
+            for (int64_t j = 0; j < D.c1 - D.c0 + 1; j++)
+                colidx.push_back(static_cast<int32_t>(D.c0 - 1 + j));
         }
+        for (int32_t b : ll) {
+            const LowRankBlock& L = h.lowrank[b];
+            for (int64_t j = 0; j < L.k; j++)
+                colidx.push_back(static_cast<int32_t>(h.n + trow_of_block[b] + j));
+        }
+        while ((colidx.size() - c0) % TK) colidx.push_back(static_cast<int32_t>(zero_row));
+        return std::make_pair(c0, static_cast<int32_t>((colidx.size() - c0) / TK));
+    };
+    auto add_panel = [&](const std::vector<int32_t>& dl, const std::vector<int32_t>& ll,
+                         const Unit& U) {
+        const int64_t c0 = static_cast<int64_t>(cdesc.size());
+        for (int32_t d : dl) {
+            const DenseBlock& D = h.dense[d];
+            const int64_t nn = D.c1 - D.c0 + 1;
+            for (int64_t j = 0; j < nn; j++) {
+                if (builtin)
+                    cdesc.push_back({h.smap[D.c0 - 1 + j] - 1, 0, 0});
+                else
+                    cdesc.push_back({D.off + (U.r0 - (D.r0 - 1)) * nn + j, static_cast<int32_t>(nn), 2});
+            }
+        }
+        for (int32_t b : ll) {
+            const LowRankBlock& L = h.lowrank[b];
+            for (int64_t j = 0; j < L.k; j++)
+                cdesc.push_back({L.uoff + (U.r0 - (L.r0 - 1)) * L.k + j, static_cast<int32_t>(L.k), 1});
+        }
+        while ((cdesc.size() - c0) % TK) cdesc.push_back({0, 0, 3});
+        const int64_t K = static_cast<int64_t>(cdesc.size()) - c0;
+        Item g{};
+        g.data_off = data_off;
+        g.col_off = c0;
+        g.out_off = U.r0;
+        g.rows = U.rows;
+        g.ktiles = static_cast<int32_t>(K / TK);
+        g.ncols = static_cast<int32_t>(K);
+        if (K > 0) gensB.push_back(g);
+        const int64_t off = data_off;
+        data_off += K * U.rows;
+        return off;
+    };
+    for (const auto& gp : groups_b) {
+        PairItem it{};
+        const Unit& A = units[gp.first];
+        it.out_off[0] = A.r0;
+        it.rows[0] = A.rows;
+        if (gp.second < 0) {
+            auto [c0, kt] = add_colidx(dense_of[A.leaf], lr_of[A.leaf]);
+            it.pr_col[0] = c0;
+            it.pr_kt[0] = kt;
+            it.pr_data[0] = add_panel(dense_of[A.leaf], lr_of[A.leaf], A);
+        } else {
+            const Unit& B = units[gp.second];
+            it.out_off[1] = B.r0;
+            it.rows[1] = B.rows;
+            const auto sd = intersect(dense_of[A.leaf], dense_of[B.leaf]);
+            const auto sl = intersect(lr_of[A.leaf], lr_of[B.leaf]);
+            auto [sc, skt] = add_colidx(sd, sl);
+            it.sh_col = sc;
+            it.sh_kt = skt;
+            it.sh_data[0] = add_panel(sd, sl, A);
+            it.sh_data[1] = add_panel(sd, sl, B);
+            const Unit* UU[2] = {&A, &B};
+            for (int q = 0; q < 2; q++) {
+                const auto pd = minus(dense_of[UU[q]->leaf], sd);
+                const auto pl = minus(lr_of[UU[q]->leaf], sl);
+                auto [pc, pkt] = add_colidx(pd, pl);
+                it.pr_col[q] = pc;
+                it.pr_kt[q] = pkt;
+                it.pr_data[q] = add_panel(pd, pl, *UU[q]);
+            }
+        }
+        itemsB.push_back(it);
     }
     op->panel_elems = data_off;
     op->npartial = npartial;
@@ -308,15 +415,15 @@ void device_build(HMatrix& h, int storage) {
         }
         dcd = upload(cdesc);
         dA = upload(itemsA);
-        dB = upload(itemsB);
+        dB = upload(gensB);
         dgen = upload(gensA);
         dvrow = upload(vrow);
         const int kid = h.src.kernel_id;
         auto gen = [&](auto tag) {
             using TS = decltype(tag);
             TS* P = static_cast<TS*>(op->panel);
-            if (!itemsB.empty())
-                hipLaunchKernelGGL(k_gen_leaf<TS>, dim3(itemsB.size()), dim3(256), 0, 0, dB, P, dcd,
+            if (!gensB.empty())
+                hipLaunchKernelGGL(k_gen_leaf<TS>, dim3(gensB.size()), dim3(256), 0, 0, dB, P, dcd,
                                    op->tmap0, dxt, dxs, dU, dD, kid);
             if (!itemsA.empty())
                 hipLaunchKernelGGL(k_gen_far<TS>, dim3(itemsA.size()), dim3(256), 0, 0, dA, dgen, P,

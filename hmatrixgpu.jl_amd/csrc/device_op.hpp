@@ -22,7 +22,7 @@ struct DeviceOp {
     int64_t m = 0, n = 0, R = 0;
     void* panel = nullptr;
     int64_t panel_elems = 0;
-    Item* itemsB = nullptr;
+    PairItem* itemsB = nullptr;
     int64_t nB = 0;
     int32_t* colidx = nullptr;
     int64_t ncol = 0;
@@ -34,7 +34,7 @@ struct DeviceOp {
     int32_t* tmap0 = nullptr;
     int32_t* inv_smap = nullptr;
     // host copies of the work lists (the batched operator re-packs them)
-    std::vector<Item> hItemsB;
+    std::vector<PairItem> hItemsB;
     std::vector<FarItem> hItemsA;
     std::vector<ReduceGroup> hGroups;
     std::mutex mu;
@@ -78,7 +78,7 @@ struct StageArrays {
     int64_t nA = 0;
     const ReduceGroup* groups = nullptr;
     int64_t nG = 0;
-    const Item* itemsB = nullptr;
+    const PairItem* itemsB = nullptr;
     int64_t nB = 0;
     int64_t gather_n = 0;   // largest source size over the instances
     int gather_insts = 1;   // instances (grid.y of the gather)
@@ -133,6 +133,6 @@ void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* in
 }
 
 // Dispatch order of leaf work items (see device.hip).
-void order_leaf_items(std::vector<Item>& items, const char* order);
+void order_leaf_items(std::vector<PairItem>& items, const char* order);
 
 }  // namespace hmx

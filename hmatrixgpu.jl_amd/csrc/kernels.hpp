@@ -45,6 +45,23 @@ struct Item {
     int32_t inst;      // instance (H-matrix) index in a batched launch
 };
 
+// Leaf work item: one or two consecutive row groups (<= 64 rows each) of the
+// target order.  The columns both row groups need (blocks whose row range
+// covers both) form the shared segment, stored as one panel per row group over
+// the same column indices; the remaining columns of each row group form its
+// private segment.  A single row group (rows[1] == 0) has only a private part.
+struct PairItem {
+    int64_t out_off[2];  // first permuted target row of each row group
+    int64_t sh_data[2];  // shared-segment panel of each row group
+    int64_t pr_data[2];  // private-segment panel of each row group
+    int64_t sh_col;      // colidx offset of the shared columns
+    int64_t pr_col[2];   // colidx offset of each private part
+    int32_t rows[2];
+    int32_t sh_kt;       // k-tiles of the shared segment
+    int32_t pr_kt[2];    // k-tiles of each private part
+    int32_t inst;        // instance (H-matrix) index in a batched launch
+};
+
 struct ReduceGroup {
     int64_t pbase;  // first partial row
     int64_t tbase;  // first T row (S row = n + tbase)
@@ -235,11 +252,21 @@ __device__ inline SRow<T, W> load_srow(__amdgpu_buffer_rsrc_t R, uint32_t off) {
 // on the VALU, lane (row li, k-chunk lg).  Loads are issued one k-tile ahead (A
 // and B) and two ahead (column indices).  CP is the cache policy of the
 // streamed A loads.
-template <typename TS, int W, bool LEAF, int NRT, int CP, typename Emit>
-__device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles, int64_t col_off,
-                                           int ncols, const int32_t* colidx,
-                                           const typename SType<TS, W>::type* S, uint32_t s_bytes,
-                                           int kt0, int kstep, int lane, Emit&& emit) {
+// Accumulator of one wave over NRT 16-row tiles: MFMA C/D registers (W = 16)
+// or per-lane partial sums (W = 1, 4).
+template <typename TS, int W>
+using PanelAcc = typename std::conditional<
+    W == 16,
+    typename std::conditional<std::is_same<typename Store<TS>::acc, double>::value, f64x4,
+                              f32x4>::type,
+    SRow<typename Store<TS>::acc, W>>::type;
+
+template <typename TS, int W, bool LEAF, int NRT, int CP>
+__device__ __forceinline__ void panel_accum(const TS* pdata, int rows, int ktiles, int64_t col_off,
+                                            int ncols, const int32_t* colidx,
+                                            const typename SType<TS, W>::type* S, uint32_t s_bytes,
+                                            int kt0, int kstep, int lane,
+                                            PanelAcc<TS, W> (&acc)[NRT]) {
     using acc_t = typename Store<TS>::acc;
     using s_t = typename SType<TS, W>::type;
     constexpr int TK = Store<TS>::TK, EPL = Store<TS>::EPL;
@@ -289,13 +316,6 @@ __device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles
         }
         return st;
     };
-
-    using vacc = typename std::conditional<W == 16,
-        typename std::conditional<std::is_same<acc_t, double>::value, f64x4, f32x4>::type,
-        SRow<acc_t, W>>::type;
-    vacc acc[NRT];
-#pragma unroll
-    for (int rt = 0; rt < NRT; rt++) acc[rt] = vacc{};
 
     auto compute = [&](const Stage& st) {
 #pragma unroll
@@ -370,7 +390,13 @@ __device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles
         compute(sB);
         kt += kstep;
     }
+}
 
+// Hand the accumulated tile results to emit(row, col, value), row < 16*NRT.
+template <typename TS, int W, int NRT, typename Emit>
+__device__ __forceinline__ void panel_emit(PanelAcc<TS, W> (&acc)[NRT], int lane, Emit&& emit) {
+    using acc_t = typename Store<TS>::acc;
+    const int li = lane & 15, lg = lane >> 4;
     if constexpr (W == 16) {
         // MFMA 16x16 output, column li: f32 accumulators hold rows 4*lg+v of the
         // row tile, the f64 instruction holds rows 4*v+lg.
@@ -394,6 +420,20 @@ __device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles
     }
 }
 
+// One wave's share of a panel, accumulated from zero and emitted.
+template <typename TS, int W, bool LEAF, int NRT, int CP, typename Emit>
+__device__ __forceinline__ void panel_body(const TS* pdata, int rows, int ktiles, int64_t col_off,
+                                           int ncols, const int32_t* colidx,
+                                           const typename SType<TS, W>::type* S, uint32_t s_bytes,
+                                           int kt0, int kstep, int lane, Emit&& emit) {
+    PanelAcc<TS, W> acc[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++) acc[rt] = PanelAcc<TS, W>{};
+    panel_accum<TS, W, LEAF, NRT, CP>(pdata, rows, ktiles, col_off, ncols, colidx, S, s_bytes, kt0,
+                                      kstep, lane, acc);
+    panel_emit<TS, W, NRT>(acc, lane, emit);
+}
+
 template <typename F>
 __device__ __forceinline__ void dispatch_nrt(int rows, F&& f) {
     switch ((rows + 15) >> 4) {
@@ -413,35 +453,58 @@ __device__ __forceinline__ const InstDesc& inst_of(const InstDesc& one, const In
         return one;
 }
 
-// Leaf panel: the kWaves waves split the k-tiles of one item; partial sums are
-// reduced through LDS and scattered to Y through target_map.
+// Leaf item: for a pair of row groups, waves 0-1 own row group 0 and waves 2-3
+// row group 1; the two waves of a row group split the k-tiles of its shared
+// and then its private panel, keeping one set of accumulators.  Waves 0 and 2
+// (1 and 3) walk the same shared k-tiles at the same time, so each shared B
+// row is fetched once per workgroup and served to the second wave from L1.  A
+// single row group is split over all four waves.  Partial sums are reduced
+// through LDS and scattered to Y through target_map.
 template <typename TS, int W, int CP, typename TIO, bool BATCH>
-__global__ __launch_bounds__(kWaves * 64) void k_leaf(const Item* __restrict__ items,
+__global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict__ items,
                                                       const InstDesc one,
                                                       const InstDesc* __restrict__ insts) {
     using acc_t = typename Store<TS>::acc;
     using s_t = typename SType<TS, W>::type;
-    const Item it = items[blockIdx.x];
+    const PairItem it = items[blockIdx.x];
     const InstDesc& I = inst_of<BATCH>(one, insts, it.inst);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rows = it.rows;
+    const bool pair = it.rows[1] > 0;
+    const int g = pair ? wave >> 1 : 0;
+    const int kt0 = pair ? (wave & 1) : wave, kstep = pair ? 2 : 4;
+    const int rows = it.rows[g];
     __shared__ acc_t red[kWaves][64][W];
     auto emit = [&](int row, int j, acc_t v) { red[wave][row][j] = v; };
+    const TS* panel = static_cast<const TS*>(I.panel);
+    const s_t* S = static_cast<const s_t*>(I.S);
     dispatch_nrt(rows, [&](auto nrt) {
-        panel_body<TS, W, true, decltype(nrt)::value, CP>(
-            static_cast<const TS*>(I.panel) + it.data_off, rows, it.ktiles, it.col_off, 0,
-            I.colidx, static_cast<const s_t*>(I.S), I.s_bytes, wave, kWaves, lane, emit);
+        constexpr int NRT = decltype(nrt)::value;
+        PanelAcc<TS, W> acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++) acc[rt] = PanelAcc<TS, W>{};
+        panel_accum<TS, W, true, NRT, CP>(panel + it.sh_data[g], rows, it.sh_kt, it.sh_col, 0,
+                                          I.colidx, S, I.s_bytes, kt0, kstep, lane, acc);
+        panel_accum<TS, W, true, NRT, CP>(panel + it.pr_data[g], rows, it.pr_kt[g], it.pr_col[g], 0,
+                                          I.colidx, S, I.s_bytes, kt0, kstep, lane, acc);
+        panel_emit<TS, W, NRT>(acc, lane, emit);
     });
     __syncthreads();
     TIO* Y = static_cast<TIO*>(I.Y);
-    for (int t = threadIdx.x; t < rows * W; t += kWaves * 64) {
-        const int r = t / W, j = t % W;
-        acc_t s = red[0][r][j];
+    const int r0 = it.rows[0], total = r0 + it.rows[1];
+    for (int t = threadIdx.x; t < total * W; t += kWaves * 64) {
+        const int rr = t / W, j = t % W;
+        const int gg = rr < r0 ? 0 : 1, r = rr - gg * r0;
+        acc_t s;
+        if (pair) {
+            s = red[2 * gg][r][j] + red[2 * gg + 1][r][j];
+        } else {
+            s = red[0][r][j];
 #pragma unroll
-        for (int w = 1; w < kWaves; w++) s += red[w][r][j];
+            for (int w = 1; w < kWaves; w++) s += red[w][r][j];
+        }
         if (I.rhs0 + j < I.nrhs)
-            Y[static_cast<int64_t>(I.tmap0[it.out_off + r]) + static_cast<int64_t>(I.rhs0 + j) * I.ldy] =
-                static_cast<TIO>(s);
+            Y[static_cast<int64_t>(I.tmap0[it.out_off[gg] + r]) +
+              static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(s);
     }
 }
 
