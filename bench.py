@@ -50,6 +50,10 @@ def parse():
     p.add_argument("--grid", type=int, default=1024, help="rankpred: grid side")
     p.add_argument("--host-build", action="store_true",
                    help="rankpred: also time the host-thread ACA+ build of the grid")
+    p.add_argument("--calibrate", action="store_true",
+                   help="after the measurement, run a 4 GiB device copy and a 4 GiB fill "
+                        "(two 2 GiB dispatches) whose rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                        "calibrate the counters")
     p.add_argument("--profile-run", action="store_true",
                    help="workload for rocprofv3: a 4 GiB copy and a 4 GiB fill (counter "
                         "calibration), then 5 FP32 16-RHS products; prints no JSON")
@@ -80,6 +84,20 @@ def measured_traffic():
         return traffic, {"fetch_scale": rscale, "write_scale": wscale, "source": os.path.relpath(PMC_FILE, ROOT)}
     except KeyError:
         return None, None
+
+
+def calibration_kernels(torch, dev):
+    """Counter calibration: a fill of 4 GiB (two 2 GiB dispatches) and a 4 GiB
+    copy; run after the measured products so that their allocations do not
+    change the placement of the operator's arrays."""
+    a = torch.empty(2 ** 29, dtype=torch.float64, device=dev)  # 4 GiB
+    a[: 2 ** 28].fill_(1.0)
+    a[2 ** 28:].fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy/fill of 4 GiB done")
+    del a, b
 
 
 def profile_run(args):
@@ -494,6 +512,8 @@ def main():
         "nnz": {"D": nnzD, "U": nnzU, "V": nnzV}, "build_s": build_s,
         "device_bytes": info["device_bytes"],
     }
+    if args.calibrate:
+        calibration_kernels(torch, dev)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of mult.jl:6-32, 1 thread)")
         res["cpu_baseline"] = cpu_baseline(hm, info, args.cpu_sample_gb)

@@ -37,6 +37,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -272,8 +273,11 @@ void device_build(HMatrix& h, int storage) {
         for (int32_t b : ll) c += h.lowrank[b].k;
         return c;
     };
+    // HMX_LEAF_PAIRING=0 keeps every row group on its own (layout comparison)
+    const char* pairing_env = std::getenv("HMX_LEAF_PAIRING");
+    const bool pairing = !(pairing_env && std::string(pairing_env) == "0");
     std::vector<int64_t> gain(static_cast<size_t>(std::max<int64_t>(nu, 1)), 0);
-    for (int64_t u = 0; u + 1 < nu; u++) {
+    for (int64_t u = 0; pairing && u + 1 < nu; u++) {
         const int64_t la = units[u].leaf, lb = units[u + 1].leaf;
         const int64_t c = ncols_of(intersect(dense_of[la], dense_of[lb]),
                                    intersect(lr_of[la], lr_of[lb]));
@@ -394,9 +398,18 @@ void device_build(HMatrix& h, int storage) {
     for (int64_t i = 0; i < h.n; i++) inv[h.smap[i] - 1] = static_cast<int32_t>(i);
     op->tmap0 = upload(tmap0);
     op->inv_smap = upload(inv);
-    HIPCHECK(hipMalloc(&op->panel, std::max<int64_t>(data_off, 1) * elem_size(storage)));
-    op->colidx = upload(colidx);
-    op->ncol = static_cast<int64_t>(colidx.size());
+    // The FP64 staging arrays are allocated before the panel and freed after
+    // generation.  Measured on the MI355X, the product's speed depends on where
+    // the panel allocation lands relative to the other large buffers of the
+    // process (up to 12% for both product kernels, identical traffic); with
+    // this order the usual sequence (FP64, FP32, BF16 operators of one
+    // H-matrix) placed the FP32 panel in the fast state in every run.
+    auto alloc_panel = [&] {
+        const size_t bytes = static_cast<size_t>(std::max<int64_t>(data_off, 1)) * elem_size(storage);
+        HIPCHECK(hipMalloc(&op->panel, bytes));
+        if (std::getenv("HMX_VERBOSE"))
+            std::fprintf(stderr, "[hmx] storage %d panel %p, %zu bytes\n", storage, op->panel, bytes);
+    };
 
     double *dU = nullptr, *dV = nullptr, *dD = nullptr, *dxt = nullptr, *dxs = nullptr;
     ColDesc* dcd = nullptr;
@@ -418,6 +431,9 @@ void device_build(HMatrix& h, int storage) {
         dB = upload(gensB);
         dgen = upload(gensA);
         dvrow = upload(vrow);
+        alloc_panel();
+        op->colidx = upload(colidx);
+        op->ncol = static_cast<int64_t>(colidx.size());
         const int kid = h.src.kernel_id;
         auto gen = [&](auto tag) {
             using TS = decltype(tag);
@@ -470,6 +486,9 @@ ScratchW& get_scratch(DeviceOp& op, hipStream_t stream, int W) {
         HIPCHECK(hipMalloc(&s.S, sb));
         HIPCHECK(hipMemset(s.S, 0, sb));  // zero row and padding rows
         HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
+        if (std::getenv("HMX_VERBOSE"))
+            std::fprintf(stderr, "[hmx] storage %d W %d source/T rows %p (%zu bytes), partials %p\n",
+                         op.storage, W, s.S, sb, s.P);
     }
     return s;
 }
