@@ -13,9 +13,10 @@
 // master copy for Adam), all accumulation is FP32.  Loss: mean squared error.
 //
 // Kernels:
-//   k_mlp_grad    a one-wave workgroup takes 64 samples (one per lane):
-//                 forward and backward in registers, then the weight gradients
-//                 dW2 = dA2^T H1 and dW1 = dA1^T X as LDS-tiled sums over the
+//   k_mlp_grad    a 256-thread workgroup takes 32 samples: activations and
+//                 their gradients live in padded LDS tiles (8 threads per
+//                 sample for the 64-wide layers), the weight gradients
+//                 dW2 = dA2^T H1 and dW1 = dA1^T X are summed over the
 //                 workgroup's samples in sample order; one partial gradient
 //                 row per workgroup.
 //   k_mlp_adam    thread per parameter: sums the partial rows in workgroup
@@ -33,7 +34,8 @@ namespace {
 
 constexpr int kIn = 6;
 constexpr int kH = 64;
-constexpr int kGradThreads = kH;  // one wave, one sample per lane
+constexpr int kGradThreads = 256;  // threads of a gradient workgroup
+constexpr int kGS = 32;            // samples per gradient workgroup
 constexpr int kPad = kH + 1;  // LDS row stride (conflict-free per-sample rows)
 constexpr int kNParams = kIn * kH + kH + kH * kH + kH + kH + 1;
 // parameter offsets
@@ -66,6 +68,10 @@ __device__ inline void normalise(const float* __restrict__ x, const Norm& nm, fl
     for (int i = 0; i < kIn; i++) xn[i] = (x[i] - nm.mean[i]) * nm.inv_std[i];
 }
 
+// W2 in LDS with a padded row stride (65): lanes reading W2[a][b] for
+// different rows a hit different banks.
+__device__ inline float w2_at(const float* lw2, int a, int b) { return lw2[a * kPad + b]; }
+
 __global__ __launch_bounds__(kGradThreads) void k_mlp_grad(const float* __restrict__ wq,
                                                            const float* __restrict__ X,
                                                            const float* __restrict__ T,
@@ -74,111 +80,131 @@ __global__ __launch_bounds__(kGradThreads) void k_mlp_grad(const float* __restri
                                                            float inv_batch, Norm nm,
                                                            float* __restrict__ part,
                                                            float* __restrict__ loss_part) {
-    __shared__ float lw[kNParams];
-    __shared__ float h1s[kGradThreads * kPad];  // H1 rows, later dA1 rows
-    __shared__ float d2s[kGradThreads * kPad];  // dA2 rows, later dz * H2 rows
-    __shared__ float xs[kGradThreads * kIn];
-    __shared__ float dzs[kGradThreads];
-    __shared__ float lred[kGradThreads];
-    load_weights(wq, lw);
-    __syncthreads();
+    __shared__ float lw[kNParams];         // W1, b1, (W2 unused here), b2, W3, b3
+    __shared__ float lw2[kH * kPad];       // W2, padded rows
+    __shared__ float xs[kGS][kIn];
+    __shared__ float h1[kGS][kPad];        // relu(A1)
+    __shared__ float a2[kGS][kPad];        // A2, later dA1
+    __shared__ float d2[kGS][kPad];        // dA2
+    __shared__ float zr[kGS][8];
+    __shared__ float dzs[kGS], lred[kGS];
     const int t = threadIdx.x;
-    const int64_t s = static_cast<int64_t>(blockIdx.x) * kGradThreads + t;
-    const bool live = s < count;
-    float xn[kIn] = {};
+    for (int p = t; p < kNParams; p += kGradThreads) lw[p] = wq[p];
+    for (int p = t; p < kH * kH; p += kGradThreads) lw2[(p / kH) * kPad + p % kH] = wq[oW2 + p];
     float target = 0.f;
-    if (live) {
-        const int64_t row = idx ? idx[first + s] : first + s;
-        normalise(X + row * kIn, nm, xn);
-        target = T[row];
-    }
-    // forward
-    float h1[kH];
-    uint64_t m1 = 0, m2 = 0;
+    bool live = false;
+    if (t < kGS) {
+        const int64_t sidx = static_cast<int64_t>(blockIdx.x) * kGS + t;
+        live = sidx < count;
+        float xn[kIn] = {};
+        if (live) {
+            const int64_t row = idx ? idx[first + sidx] : first + sidx;
+            normalise(X + row * kIn, nm, xn);
+            target = T[row];
+        }
 #pragma unroll
-    for (int a = 0; a < kH; a++) {
+        for (int i = 0; i < kIn; i++) xs[t][i] = xn[i];
+    }
+    __syncthreads();
+    // forward: thread -> sample s, eight output units a0 .. a0 + 7
+    const int s = t >> 3, a0 = (t & 7) * 8;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int a = a0 + c;
         float acc = lw[oB1 + a];
 #pragma unroll
-        for (int i = 0; i < kIn; i++) acc += lw[oW1 + a * kIn + i] * xn[i];
-        if (acc > 0.f) m1 |= uint64_t(1) << a;
-        h1[a] = acc > 0.f ? acc : 0.f;
+        for (int i = 0; i < kIn; i++) acc += lw[oW1 + a * kIn + i] * xs[s][i];
+        h1[s][a] = acc > 0.f ? acc : 0.f;
     }
-    float z = lw[oB3];
-    float h2[kH];
+    __syncthreads();
+    float zp = 0.f;
 #pragma unroll
-    for (int a = 0; a < kH; a++) {
+    for (int c = 0; c < 8; c++) {
+        const int a = a0 + c;
         float acc = lw[oB2 + a];
-#pragma unroll
-        for (int b = 0; b < kH; b++) acc += lw[oW2 + a * kH + b] * h1[b];
-        if (acc > 0.f) m2 |= uint64_t(1) << a;
-        h2[a] = acc > 0.f ? acc : 0.f;
-        z += lw[oW3 + a] * h2[a];
+#pragma unroll 8
+        for (int b = 0; b < kH; b++) acc += w2_at(lw2, a, b) * h1[s][b];
+        a2[s][a] = acc;
+        zp += lw[oW3 + a] * (acc > 0.f ? acc : 0.f);
     }
-    const float diff = live ? z - target : 0.f;
-    const float dz = 2.f * diff * inv_batch;
-    // backward: dA2 (to LDS) and dH1 = W2^T dA2
-    float d1[kH];
+    zr[s][t & 7] = zp;
+    __syncthreads();
+    if (t < kGS) {
+        float z = lw[oB3];
 #pragma unroll
-    for (int b = 0; b < kH; b++) d1[b] = 0.f;
-#pragma unroll
-    for (int a = 0; a < kH; a++) {
-        const float da2 = (m2 >> a & 1) ? dz * lw[oW3 + a] : 0.f;
-        d2s[t * kPad + a] = da2;
-#pragma unroll
-        for (int b = 0; b < kH; b++) d1[b] += lw[oW2 + a * kH + b] * da2;
+        for (int q = 0; q < 8; q++) z += zr[t][q];
+        const float diff = live ? z - target : 0.f;
+        dzs[t] = 2.f * diff * inv_batch;
+        lred[t] = diff * diff;
     }
+    __syncthreads();
+    {  // dA2 = dz * W3 where A2 > 0
+        const float dz = dzs[s];
 #pragma unroll
-    for (int b = 0; b < kH; b++) h1s[t * kPad + b] = h1[b];
-#pragma unroll
-    for (int i = 0; i < kIn; i++) xs[t * kIn + i] = xn[i];
-    dzs[t] = dz;
-    lred[t] = diff * diff;
+        for (int c = 0; c < 8; c++) {
+            const int a = a0 + c;
+            d2[s][a] = a2[s][a] > 0.f ? dz * lw[oW3 + a] : 0.f;
+        }
+    }
     __syncthreads();
     float* g = part + static_cast<int64_t>(blockIdx.x) * kNParams;
-    {  // dW2[a][:] = sum_r dA2[r][a] * H1[r][:] and db2[a], a = t, samples in order
-        float acc[kH];
+    {  // dW2[a][b0 .. b0+15] and db2[a] over the workgroup's samples, in order
+        const int a = t >> 2, b0 = (t & 3) * 16;
+        float acc[16];
 #pragma unroll
-        for (int c = 0; c < kH; c++) acc[c] = 0.f;
+        for (int c = 0; c < 16; c++) acc[c] = 0.f;
         float db2 = 0.f;
-        for (int r = 0; r < kGradThreads; r++) {
-            const float d = d2s[r * kPad + t];
+#pragma unroll 4
+        for (int r = 0; r < kGS; r++) {
+            const float d = d2[r][a];
             db2 += d;
 #pragma unroll
-            for (int c = 0; c < kH; c++) acc[c] += d * h1s[r * kPad + c];
+            for (int c = 0; c < 16; c++) acc[c] += d * h1[r][b0 + c];
         }
 #pragma unroll
-        for (int c = 0; c < kH; c++) g[oW2 + t * kH + c] = acc[c];
-        g[oB2 + t] = db2;
+        for (int c = 0; c < 16; c++) g[oW2 + a * kH + b0 + c] = acc[c];
+        if ((t & 3) == 0) g[oB2 + a] = db2;
     }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < kH; a++) d2s[t * kPad + a] = dz * h2[a];
-#pragma unroll
-    for (int b = 0; b < kH; b++) h1s[t * kPad + b] = (m1 >> b & 1) ? d1[b] : 0.f;
-    __syncthreads();
-    {  // dW3[t], db1[t], dW1[t][:]
-        float w3 = 0.f, db1 = 0.f;
-        float w1[kIn] = {};
-        for (int r = 0; r < kGradThreads; r++) {
-            w3 += d2s[r * kPad + t];
-            const float d = h1s[r * kPad + t];
-            db1 += d;
-#pragma unroll
-            for (int i = 0; i < kIn; i++) w1[i] += d * xs[r * kIn + i];
-        }
+    if (t < kH) {  // dW3[a] = sum_s dz * relu(A2[s][a])
+        float w3 = 0.f;
+        for (int r = 0; r < kGS; r++) w3 += dzs[r] * (a2[r][t] > 0.f ? a2[r][t] : 0.f);
         g[oW3 + t] = w3;
-        g[oB1 + t] = db1;
-#pragma unroll
-        for (int i = 0; i < kIn; i++) g[oW1 + t * kIn + i] = w1[i];
-    }
-    if (t == 0) {
+    } else if (t == kH) {
         float db3 = 0.f, l = 0.f;
-        for (int r = 0; r < kGradThreads; r++) {
+        for (int r = 0; r < kGS; r++) {
             db3 += dzs[r];
             l += lred[r];
         }
         g[oB3] = db3;
         loss_part[blockIdx.x] = l;
+    }
+    __syncthreads();
+    {  // dA1 = (W2^T dA2) where A1 > 0, written over A2
+        float acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = 0.f;
+#pragma unroll 4
+        for (int a = 0; a < kH; a++) {
+            const float d = d2[s][a];
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] += w2_at(lw2, a, a0 + c) * d;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) a2[s][a0 + c] = h1[s][a0 + c] > 0.f ? acc[c] : 0.f;
+    }
+    __syncthreads();
+    if (t < kH) {  // dW1[a][:], db1[a]
+        float db1 = 0.f;
+        float w1[kIn] = {};
+        for (int r = 0; r < kGS; r++) {
+            const float d = a2[r][t];
+            db1 += d;
+#pragma unroll
+            for (int i = 0; i < kIn; i++) w1[i] += d * xs[r][i];
+        }
+        g[oB1 + t] = db1;
+#pragma unroll
+        for (int i = 0; i < kIn; i++) g[oW1 + t * kIn + i] = w1[i];
     }
 }
 
@@ -339,7 +365,7 @@ static void set_norm_from(RankPredictor& p, const float* X, int64_t n) {
 static int grad_pass(RankPredictor& p, const float* dX, const float* dT, const int32_t* didx,
                      int64_t first, int64_t count, float* part, float* loss_part, float* dloss,
                      float lr, bool apply, hipStream_t st) {
-    const int nparts = static_cast<int>((count + kGradThreads - 1) / kGradThreads);
+    const int nparts = static_cast<int>((count + kGS - 1) / kGS);
     hipLaunchKernelGGL(k_mlp_grad, dim3(nparts), dim3(kGradThreads), 0, st, p.wq, dX, dT, didx,
                        first, count, 1.f / static_cast<float>(count), p.norm, part, loss_part);
     HIPCHECK(hipGetLastError());
@@ -364,7 +390,7 @@ void rankpred_train(RankPredictor& p, const float* X, const float* T, int64_t n,
     float* dX = dalloc<float>(n * kIn);
     float* dT = dalloc<float>(n);
     int32_t* didx = dalloc<int32_t>(n);
-    const int maxparts = (batch + kGradThreads - 1) / kGradThreads;
+    const int maxparts = (batch + kGS - 1) / kGS;
     float* part = dalloc<float>(static_cast<int64_t>(maxparts) * kNParams);
     float* lpart = dalloc<float>(maxparts);
     float* dloss = dalloc<float>(1);
@@ -408,7 +434,7 @@ void rankpred_grad(RankPredictor& p, const float* X, const float* T, int64_t n, 
     if (!p.norm_set) set_norm_from(p, X, n);
     float* dX = dalloc<float>(n * kIn);
     float* dT = dalloc<float>(n);
-    const int nparts = static_cast<int>((n + kGradThreads - 1) / kGradThreads);
+    const int nparts = static_cast<int>((n + kGS - 1) / kGS);
     float* part = dalloc<float>(static_cast<int64_t>(nparts) * kNParams);
     float* lpart = dalloc<float>(nparts);
     float* dloss = dalloc<float>(1);
