@@ -159,3 +159,17 @@ def test_vector_kernel_dims3_callback():
 
     Hp = H.HMatrix(Dense(), t_p, s_p, eta=1.5, eps=1e-6, storage=())
     _layouts_equal(Ho, Hp)
+
+
+@pytest.mark.parametrize("n,seed", [(65536, 2), (262144, 3)])
+def test_full_size_tree_and_block_lists_bit_exact(n, seed):
+    """Configs 2 and 3 at their full sizes (leaf 64, eta 2): the permutation, the
+    cluster tree and the dense / admissible block lists equal the oracle's."""
+    pts = np.random.default_rng(seed).random((n, 3)).T.copy()  # the bench's point sets
+    t_o, t_p = _same_tree(pts, 64)
+    b_o = O.BlockTree(t_o, t_o, 2.0, merge=True)
+    b_p = H.BlockTree(t_p, t_p, eta=2.0)
+    H.merge_dense_matrices_(b_p.root)
+    d_o, a_o = b_o.traverse()
+    d_p, a_p = b_p.lists()
+    assert np.array_equal(d_o, d_p) and np.array_equal(a_o, a_p)
