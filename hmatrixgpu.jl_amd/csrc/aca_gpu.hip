@@ -215,6 +215,241 @@ __global__ __launch_bounds__(256) void k_copy_segs(const Seg* __restrict__ segs,
     for (int64_t i = threadIdx.x; i < s.len; i += 256) to[s.dst + i] = from[s.src + i];
 }
 
+// ---- recompression (SVD_recompress, aca.jl:123-143) on the device ------------------
+// One wave per block, on the ACA+ terms still in the slot pool: Householder QR
+// of U (m x k) and V' (n x k) in place, the k x k core R_U R_V', one-sided
+// Jacobi SVD, truncation, then U' = Q_U (W_r S_r) and V = (Q_V Z_r)'.  The
+// operation order is the host's (svd_recompress): reflector sums in the 64-lane
+// order of dot_lanes64, every other sum sequential in one lane, contraction
+// off, so the factors are bit-identical to the host recompression.
+struct RecTask {
+    int64_t uoff, voff;  // ACA terms: U m x k, V' n x k, column-major, in the pool
+    int64_t soff;        // scratch: M, Z, W, Z2 (k*k each), nrm, sig, ord, tu, tv (k each)
+    int64_t ooff;        // output: U' (m x r column-major) then V (r x n column-major)
+    int32_t m, n, k, r;
+};
+
+__device__ __forceinline__ double wave_fold(double p) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) p = p + __shfl_down(p, off);
+    return __shfl(p, 0);
+}
+
+// sum_i x[i*sx] * y[i*sy], i < n, in the 64-lane order
+__device__ double wave_dot(const double* x, int64_t sx, const double* y, int64_t sy, int n,
+                           int lane) {
+#pragma clang fp contract(off)
+    double p = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const double pr = x[i * sx] * y[i * sy];
+        p = p + pr;
+    }
+    return wave_fold(p);
+}
+
+__device__ void wave_qr(int m, int k, double* A, double* tau, int lane) {
+#pragma clang fp contract(off)
+    for (int j = 0; j < k; j++) {
+        double* a = A + static_cast<int64_t>(j) * m;
+        const double alpha = a[j];
+        const double xs = wave_dot(a + j + 1, 1, a + j + 1, 1, m - j - 1, lane);
+        if (xs == 0.0) {
+            if (lane == 0) tau[j] = 0.0;
+            __syncthreads();
+            continue;
+        }
+        const double nrm = sqrt(alpha * alpha + xs);
+        const double beta = alpha >= 0.0 ? -nrm : nrm;
+        const double t = (beta - alpha) / beta;
+        const double scale = 1.0 / (alpha - beta);
+        for (int i = j + 1 + lane; i < m; i += 64) a[i] = a[i] * scale;
+        __syncthreads();
+        if (lane == 0) {
+            a[j] = beta;
+            tau[j] = t;
+        }
+        for (int c = j + 1; c < k; c++) {
+            double* b = A + static_cast<int64_t>(c) * m;
+            const double bj = b[j];
+            double w = bj + wave_dot(a + j + 1, 1, b + j + 1, 1, m - j - 1, lane);
+            w = w * t;
+            __syncthreads();
+            if (lane == 0) b[j] = bj - w;
+            for (int i = j + 1 + lane; i < m; i += 64) b[i] = b[i] - a[i] * w;
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+// B <- Q B for ncols columns; element i of column c at B[c*cs + i*es]
+__device__ void wave_apply_q(int m, int k, const double* A, const double* tau, int ncols,
+                             double* B, int64_t cs, int64_t es, int lane) {
+#pragma clang fp contract(off)
+    for (int j = k - 1; j >= 0; j--) {
+        const double t = tau[j];
+        if (t == 0.0) continue;
+        const double* a = A + static_cast<int64_t>(j) * m;
+        for (int c = 0; c < ncols; c++) {
+            double* b = B + c * cs;
+            const double bj = b[j * es];
+            double w = bj + wave_dot(a + j + 1, 1, b + (j + 1) * es, es, m - j - 1, lane);
+            w = w * t;
+            __syncthreads();
+            if (lane == 0) b[j * es] = bj - w;
+            for (int i = j + 1 + lane; i < m; i += 64) b[i * es] = b[i * es] - a[i] * w;
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, double* pool,
+                                                 double* __restrict__ scratch, double eps) {
+#pragma clang fp contract(off)
+    RecTask& T = tasks[blockIdx.x];
+    const int lane = threadIdx.x, m = T.m, n = T.n, k = T.k;
+    if (k == 0) {
+        if (lane == 0) T.r = 0;
+        return;
+    }
+    double* QU = pool + T.uoff;
+    double* QV = pool + T.voff;
+    double* M = scratch + T.soff;
+    double* Z = M + k * k;
+    double* W = Z + k * k;
+    double* Z2 = W + k * k;
+    double* nrm = Z2 + k * k;
+    double* sig = nrm + k;
+    double* ord = sig + k;
+    double* tu = ord + k;
+    double* tv = tu + k;
+    wave_qr(m, k, QU, tu, lane);
+    wave_qr(n, k, QV, tv, lane);
+    for (int idx = lane; idx < k * k; idx += 64) {  // (R_U R_V')[i, j]
+        const int i = idx % k, j = idx / k;
+        double s = 0.0;
+        for (int l = max(i, j); l < k; l++) {
+            const double pr = QU[i + static_cast<int64_t>(l) * m] * QV[j + static_cast<int64_t>(l) * n];
+            s = s + pr;
+        }
+        M[i + j * k] = s;
+        Z[idx] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // one-sided Jacobi: cyclic sweeps over p < q
+    const double tol = 2.220446049250313e-16;
+    for (int sweep = 0; sweep < 80; sweep++) {
+        bool rotated = false;
+        for (int p = 0; p + 1 < k; p++) {
+            for (int q = p + 1; q < k; q++) {
+                double* ap = M + p * k;
+                double* aq = M + q * k;
+                double v = 0.0;
+                if (lane < 3) {  // alpha, beta, gamma: one sequential sum each
+                    const double* x = lane == 1 ? aq : ap;
+                    const double* y = lane == 0 ? ap : aq;
+                    for (int i = 0; i < k; i++) {
+                        const double pr = x[i] * y[i];
+                        v = v + pr;
+                    }
+                }
+                const double al = __shfl(v, 0), be = __shfl(v, 1), ga = __shfl(v, 2);
+                if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double s = c * t;
+                double* zp = Z + p * k;
+                double* zq = Z + q * k;
+                for (int i = lane; i < k; i += 64) {
+                    const double x = ap[i], y = aq[i];
+                    ap[i] = c * x - s * y;
+                    aq[i] = s * x + c * y;
+                    const double zx = zp[i], zy = zq[i];
+                    zp[i] = c * zx - s * zy;
+                    zq[i] = s * zx + c * zy;
+                }
+                __syncthreads();
+            }
+        }
+        if (!rotated) break;
+    }
+    for (int j = lane; j < k; j += 64) {
+        double s = 0.0;
+        for (int i = 0; i < k; i++) {
+            const double pr = M[i + j * k] * M[i + j * k];
+            s = s + pr;
+        }
+        nrm[j] = sqrt(s);
+    }
+    __syncthreads();
+    if (lane == 0) {  // stable sort by descending norm
+        for (int j = 0; j < k; j++) ord[j] = j;
+        for (int j = 1; j < k; j++) {
+            const double key = ord[j];
+            int x = j - 1;
+            while (x >= 0 && nrm[static_cast<int>(ord[x])] < nrm[static_cast<int>(key)]) {
+                ord[x + 1] = ord[x];
+                x--;
+            }
+            ord[x + 1] = key;
+        }
+    }
+    __syncthreads();
+    for (int idx = lane; idx < k * k; idx += 64) {
+        const int i = idx % k, j = idx / k, o = static_cast<int>(ord[j]);
+        W[i + j * k] = nrm[o] > 0.0 ? M[i + o * k] / nrm[o] : 0.0;
+        Z2[i + j * k] = Z[i + o * k];
+    }
+    for (int j = lane; j < k; j += 64) sig[j] = nrm[static_cast<int>(ord[j])];
+    __syncthreads();
+    if (lane == 0) {  // reverse(sqrt.(cumsum(reverse(sig.^2)))), first index below eps
+        double acc = 0.0;
+        for (int i = k - 1; i >= 0; i--) {
+            const double s2 = sig[i] * sig[i];
+            acc = (i == k - 1) ? s2 : acc + s2;
+            nrm[i] = sqrt(acc);
+        }
+        int r = k;
+        for (int i = 0; i < k; i++)
+            if (nrm[i] < eps) {
+                r = i;
+                break;
+            }
+        T.r = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rec_apply(const RecTask* __restrict__ tasks,
+                                                  const double* __restrict__ pool,
+                                                  const double* __restrict__ scratch,
+                                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const RecTask T = tasks[blockIdx.x];
+    const int lane = threadIdx.x, m = T.m, n = T.n, k = T.k, r = T.r;
+    if (r == 0) return;
+    const double* W = scratch + T.soff + 2 * k * k;
+    const double* Z2 = W + k * k;
+    const double* sig = Z2 + k * k + k;
+    const double* tu = sig + 2 * k;
+    const double* tv = tu + k;
+    double* U = out + T.ooff;                        // m x r, column-major
+    double* V = U + static_cast<int64_t>(m) * r;     // r x n, column-major: (j, c) at j + c*r
+    for (int64_t idx = lane; idx < static_cast<int64_t>(m) * r; idx += 64) {
+        const int i = static_cast<int>(idx % m), j = static_cast<int>(idx / m);
+        U[idx] = i < k ? W[i + j * k] * sig[j] : 0.0;
+    }
+    for (int64_t idx = lane; idx < static_cast<int64_t>(n) * r; idx += 64) {
+        const int i = static_cast<int>(idx % n), j = static_cast<int>(idx / n);  // C(i, j)
+        V[j + static_cast<int64_t>(i) * r] = i < k ? Z2[i + j * k] : 0.0;
+    }
+    __syncthreads();
+    wave_apply_q(m, k, pool + T.uoff, tu, r, U, m, 1, lane);
+    wave_apply_q(n, k, pool + T.voff, tv, r, V, 1, r, lane);
+}
+
 int64_t round_cap(int64_t c, int64_t kmax) {
     c = std::max<int64_t>(4, (c + 3) / 4 * 4);
     return std::min(c, kmax);
@@ -225,8 +460,7 @@ int64_t round_cap(int64_t c, int64_t kmax) {
 int64_t aca_round_cap(int64_t c, int64_t kmax) { return round_cap(c, kmax); }
 
 void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<int32_t> caps,
-             double eps, int nthreads, AcaStats* stats,
-             const std::function<void(int64_t, LowRank&&)>& on_done,
+             double eps, double svd_eps, int nthreads, AcaStats* stats, const AcaDone& on_done,
              const std::vector<int64_t>& probe, const AcaRefine& refine) {
     // 1/r only: its entry (subtractions, products, sqrt, division) is correctly
     // rounded on both sides, so the device reproduces the host factors exactly;
@@ -301,6 +535,140 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<
         int64_t done_blocks = 0;
         std::vector<int64_t> seen_ids;
         std::vector<int32_t> seen_ranks;
+        // Hand finished blocks to the host: copy `tot` doubles at dsrc into a
+        // pinned staging buffer, then a worker builds the LowRank of every entry
+        // of `outs` and calls on_done.
+        struct Out {
+            int64_t block, m, n, k, aca_k, off;
+        };
+        auto stage_out = [&](std::vector<Out>&& outs, const double* dsrc, int64_t tot,
+                             bool recompressed) {
+            Staging& st = stage[next_stage];
+            next_stage ^= 1;
+            if (st.job.valid()) st.job.get();  // the buffer's previous job is done
+            if (tot > st.cap) {
+                if (st.host) HIPCHECK(hipHostFree(st.host));
+                st.host = nullptr;
+                st.cap = 0;
+                HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&st.host), tot * sizeof(double),
+                                       hipHostMallocDefault));
+                st.cap = tot;
+            }
+            if (tot > 0)
+                HIPCHECK(hipMemcpy(st.host, dsrc, tot * sizeof(double), hipMemcpyDeviceToHost));
+            const double* hbuf = st.host;
+            st.job = std::async(std::launch::async, [outs = std::move(outs), hbuf, nthreads,
+                                                     recompressed, &on_done] {
+                parallel_for(static_cast<int64_t>(outs.size()), nthreads, [&](int64_t f) {
+                    const Out& o = outs[f];
+                    LowRank L;
+                    L.m = o.m;
+                    L.n = o.n;
+                    L.k = o.k;
+                    L.U.assign(hbuf + o.off, hbuf + o.off + o.k * o.m);
+                    const double* src = hbuf + o.off + o.k * o.m;
+                    if (recompressed) {  // V already k x n column-major
+                        L.V.assign(src, src + o.k * o.n);
+                    } else {             // terms q: V row q contiguous over n
+                        L.V.resize(static_cast<size_t>(o.k * o.n));
+                        for (int64_t q = 0; q < o.k; q++)
+                            for (int64_t j = 0; j < o.n; j++) L.V[q + j * o.k] = src[q * o.n + j];
+                    }
+                    on_done(o.block, o.aca_k, std::move(L), recompressed);
+                });
+            });
+        };
+        // ACA+ factors as they are (recompression on the host)
+        auto emit_raw = [&](const std::vector<int64_t>& finished, const std::vector<AcaTask>& tasks,
+                            const std::vector<AcaState>& hstate, const double* pool,
+                            const std::vector<int64_t>& order, int64_t pos) {
+            std::vector<Out> outs;
+            std::vector<Seg> pack;
+            int64_t ptot = 0;
+            for (int64_t t : finished) {
+                const AcaTask& T = tasks[t];
+                const int64_t k = hstate[t].k;
+                outs.push_back(Out{order[pos + T.sid], T.m, T.n, k, k, ptot});
+                if (k > 0) {
+                    pack.push_back(Seg{T.uoff, ptot, k * T.m});
+                    pack.push_back(Seg{T.voff, ptot + k * T.m, k * T.n});
+                }
+                ptot += k * (T.m + T.n);
+            }
+            double* dpack = nullptr;
+            if (!pack.empty()) {
+                HIPCHECK(hipMalloc(&dpack, ptot * sizeof(double)));
+                owned.push_back(dpack);
+                Seg* dps = upload(pack);
+                owned.push_back(dps);
+                hipLaunchKernelGGL(k_copy_segs, dim3(pack.size()), dim3(256), 0, nullptr, dps, pool,
+                                   dpack);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipDeviceSynchronize());
+                release(dps);
+            }
+            stage_out(std::move(outs), dpack, ptot, false);
+            release(dpack);
+        };
+        // recompression on the device, in sub-batches of bounded scratch
+        auto emit_recompressed = [&](const std::vector<int64_t>& finished,
+                                     const std::vector<AcaTask>& tasks,
+                                     const std::vector<AcaState>& hstate, double* pool,
+                                     const std::vector<int64_t>& order, int64_t pos) {
+            const int64_t rec_budget = int64_t(1) << 28;  // doubles of scratch per sub-batch
+            size_t f0 = 0;
+            while (f0 < finished.size()) {
+                std::vector<RecTask> rt;
+                std::vector<int64_t> ids;
+                int64_t sc = 0;
+                size_t f1 = f0;
+                while (f1 < finished.size()) {
+                    const AcaTask& T = tasks[finished[f1]];
+                    const int64_t k = hstate[finished[f1]].k;
+                    const int64_t need = 4 * k * k + 5 * k;
+                    if (f1 > f0 && sc + need > rec_budget) break;
+                    rt.push_back(RecTask{T.uoff, T.voff, sc, 0, T.m, T.n, static_cast<int32_t>(k), 0});
+                    ids.push_back(finished[f1]);
+                    sc += need;
+                    f1++;
+                }
+                double* dscr = nullptr;
+                HIPCHECK(hipMalloc(&dscr, std::max<int64_t>(sc, 1) * sizeof(double)));
+                owned.push_back(dscr);
+                RecTask* drt = upload(rt);
+                owned.push_back(drt);
+                HIPCHECK(hipEventRecord(e0, nullptr));
+                hipLaunchKernelGGL(k_rec_core, dim3(rt.size()), dim3(64), 0, nullptr, drt, pool, dscr,
+                                   svd_eps);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpy(rt.data(), drt, rt.size() * sizeof(RecTask), hipMemcpyDeviceToHost));
+                std::vector<Out> outs;
+                int64_t otot = 0;
+                for (size_t q = 0; q < rt.size(); q++) {
+                    rt[q].ooff = otot;
+                    const AcaTask& T = tasks[ids[q]];
+                    outs.push_back(Out{order[pos + T.sid], T.m, T.n, rt[q].r, rt[q].k, otot});
+                    otot += int64_t(rt[q].r) * (T.m + T.n);
+                }
+                HIPCHECK(hipMemcpy(drt, rt.data(), rt.size() * sizeof(RecTask), hipMemcpyHostToDevice));
+                double* dout = nullptr;
+                HIPCHECK(hipMalloc(&dout, std::max<int64_t>(otot, 1) * sizeof(double)));
+                owned.push_back(dout);
+                hipLaunchKernelGGL(k_rec_apply, dim3(rt.size()), dim3(64), 0, nullptr, drt, pool, dscr,
+                                   dout);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipEventRecord(e1, nullptr));
+                HIPCHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                S.recompress_ms += ms;
+                stage_out(std::move(outs), dout, otot, true);
+                release(dout);
+                release(drt);
+                release(dscr);
+                f0 = f1;
+            }
+        };
         // Run the listed blocks (in this order) in memory-bounded chunks.
         auto run_list = [&](const std::vector<int64_t>& order, bool collect) {
         const int64_t nl = static_cast<int64_t>(order.size());
@@ -409,70 +777,19 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<
                 for (int64_t t : active) (hstate[t].done ? finished : cont).push_back(t);
                 if (pass == 0) S.unconverged_first += static_cast<int64_t>(cont.size());
 
-                // pack the finished blocks' terms (U then V per block) on the device
-                struct Out {
-                    int64_t block, m, n, k, off;
-                };
-                std::vector<Out> outs;
-                std::vector<Seg> pack;
-                int64_t ptot = 0;
                 for (int64_t t : finished) {
                     const AcaTask& T = tasks[t];
                     const int64_t k = hstate[t].k;
-                    outs.push_back(Out{order[pos + T.sid], T.m, T.n, k, ptot});
                     if (collect) {
                         seen_ids.push_back(order[pos + T.sid]);
                         seen_ranks.push_back(static_cast<int32_t>(k));
                     }
-                    if (k > 0) {
-                        pack.push_back(Seg{T.uoff, ptot, k * T.m});
-                        pack.push_back(Seg{T.voff, ptot + k * T.m, k * T.n});
-                    }
-                    ptot += k * (T.m + T.n);
                     S.used_elems += int64_t(T.m + T.n) * k;
                 }
-                Staging& st = stage[next_stage];
-                next_stage ^= 1;
-                if (st.job.valid()) st.job.get();  // the buffer's previous job is done
-                if (ptot > st.cap) {
-                    if (st.host) HIPCHECK(hipHostFree(st.host));
-                    st.host = nullptr;
-                    st.cap = 0;
-                    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&st.host),
-                                           ptot * sizeof(double), hipHostMallocDefault));
-                    st.cap = ptot;
-                }
-                if (!pack.empty()) {
-                    double* dpack = nullptr;
-                    HIPCHECK(hipMalloc(&dpack, ptot * sizeof(double)));
-                    owned.push_back(dpack);
-                    Seg* dps = upload(pack);
-                    owned.push_back(dps);
-                    hipLaunchKernelGGL(k_copy_segs, dim3(pack.size()), dim3(256), 0, nullptr, dps,
-                                       pool, dpack);
-                    HIPCHECK(hipGetLastError());
-                    HIPCHECK(hipMemcpy(st.host, dpack, ptot * sizeof(double),
-                                       hipMemcpyDeviceToHost));
-                    release(dps);
-                    release(dpack);
-                }
-                const double* hbuf = st.host;
-                st.job = std::async(std::launch::async, [outs = std::move(outs), hbuf, nthreads,
-                                                         &on_done] {
-                    parallel_for(static_cast<int64_t>(outs.size()), nthreads, [&](int64_t f) {
-                        const Out& o = outs[f];
-                        LowRank L;
-                        L.m = o.m;
-                        L.n = o.n;
-                        L.k = o.k;
-                        L.U.assign(hbuf + o.off, hbuf + o.off + o.k * o.m);
-                        L.V.resize(static_cast<size_t>(o.k * o.n));
-                        const double* src = hbuf + o.off + o.k * o.m;
-                        for (int64_t q = 0; q < o.k; q++)
-                            for (int64_t j = 0; j < o.n; j++) L.V[q + j * o.k] = src[q * o.n + j];
-                        on_done(o.block, std::move(L));
-                    });
-                });
+                if (svd_eps >= 0.0)
+                    emit_recompressed(finished, tasks, hstate, pool, order, pos);
+                else
+                    emit_raw(finished, tasks, hstate, pool, order, pos);
                 done_blocks += static_cast<int64_t>(finished.size());
                 for (int64_t t : cont) {
                     AcaTask& T = tasks[t];

@@ -259,6 +259,7 @@ void cache_store(const std::string& key, const BatchCfg& cfg) {
 }  // namespace
 
 Batch::~Batch() {
+    for (HMatrix* h : hs) h->batch_refs.fetch_sub(1);
     (void)hipSetDevice(device);
     for (auto& e : slot_ev)
         if (e) {
@@ -325,6 +326,7 @@ int hmx_batch_create(hmx_hmatrix* const* hs, int32_t count, int32_t storage, int
             if (i > 0 && h->device != b->device)
                 throw hmx::Error(HMX_ERR_ARG, "batch members on different devices");
             b->device = h->device;
+            h->batch_refs.fetch_add(1);  // released by ~Batch
             b->hs.push_back(h);
             b->ops.push_back(h->dev[storage].get());
             b->payload_bytes += hmx::device_panel_bytes(h->dev[storage].get());

@@ -328,6 +328,7 @@ int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t
             stats->slot_elems = ast.slot_elems;
             stats->used_elems = ast.used_elems;
             stats->aca_kernel_ms = ast.kernel_ms;
+            stats->recompress_ms = ast.recompress_ms;
             stats->aca_wall_s = ast.wall_s;
             stats->build_s = h->build_seconds;
             stats->total_s =
@@ -431,6 +432,8 @@ int hmx_hmatrix_materialize(hmx_hmatrix* hm, int32_t storage_mask, int32_t devic
         need(hm && hm->h, "hmx_hmatrix_materialize: NULL");
         need(storage_mask >= 0 && storage_mask < 8, "hmx_hmatrix_materialize: bad storage_mask");
         hmx::HMatrix& h = *hm->h;
+        if (h.batch_refs.load() > 0)
+            throw hmx::Error(HMX_ERR_STATE, "H-matrix is a member of a live batch");
         for (int s = 0; s < 3; s++)
             if ((storage_mask & (1 << s)) && h.dev[s] && device != h.device)
                 throw hmx::Error(HMX_ERR_STATE, "storage already materialised on another device");
@@ -443,6 +446,8 @@ int hmx_hmatrix_materialize(hmx_hmatrix* hm, int32_t storage_mask, int32_t devic
 int hmx_hmatrix_release(hmx_hmatrix* hm, int32_t storage_mask) {
     return guarded([&] {
         need(hm && hm->h, "hmx_hmatrix_release: NULL");
+        if (hm->h->batch_refs.load() > 0)
+            throw hmx::Error(HMX_ERR_STATE, "H-matrix is a member of a live batch");
         for (int s = 0; s < 3; s++)
             if (storage_mask & (1 << s)) hm->h->dev[s].reset();
     });
@@ -579,7 +584,11 @@ int hmx_hmatrix_info(const hmx_hmatrix* hm, double* out) {
 }
 
 int hmx_hmatrix_destroy(hmx_hmatrix* h) {
-    return guarded([&] { delete h; });
+    return guarded([&] {
+        if (h && h->h && h->h->batch_refs.load() > 0)
+            throw hmx::Error(HMX_ERR_STATE, "H-matrix is a member of a live batch");
+        delete h;
+    });
 }
 
 int hmx_matvec(hmx_hmatrix* hm, int32_t storage, const void* X, int64_t ldx, void* Y, int64_t ldy,

@@ -100,6 +100,8 @@ LowRank svd_recompress(const LowRank& in, double eps);
 // Fixed-order sum of squares used by the ACA+ stop test (host and device).
 constexpr int kSumLanes = 256;
 double sumsq_lanes(const double* x, int64_t n);
+// Dot product in the fixed 64-lane order of the QR reflectors (host and device).
+double dot_lanes64(const double* x, const double* y, int64_t n);
 
 // ---- H-matrix -----------------------------------------------------------------
 struct DenseBlock {
@@ -145,6 +147,9 @@ struct HMatrix {
         int32_t cap;       // first rank capacity of the device ACA+ (0: host ACA+)
     };
     std::vector<ApproxStat> approx_stats;
+    // batches (batch.hip) holding pointers to this H-matrix's device operators;
+    // while non-zero the operators may not be released, rebuilt or destroyed
+    std::atomic<int> batch_refs{0};
     void ensure_dense_payload();
 };
 
@@ -181,6 +186,7 @@ struct AcaStats {
     int64_t slot_elems = 0;         // sum over passes of (m + n) * capacity
     int64_t used_elems = 0;         // sum of (m + n) * final rank
     double kernel_ms = 0.0;         // ACA+ kernel time (HIP events)
+    double recompress_ms = 0.0;     // device recompression time (HIP events)
     double wall_s = 0.0;            // whole phase including copies and callbacks
 };
 // ACA+ (eps is the ACA tolerance) of every block with the given initial rank
@@ -190,9 +196,12 @@ struct AcaStats {
 // refine(ids, aca_ranks, caps) may rewrite the capacities of the other blocks.
 using AcaRefine = std::function<void(const std::vector<int64_t>&, const std::vector<int32_t>&,
                                      std::vector<int32_t>&)>;
+// on_done(block, aca_rank, factors, recompressed): factors are the ACA+ terms,
+// or (svd_eps >= 0) their device recompression at svd_eps (bit-identical to
+// svd_recompress).
+using AcaDone = std::function<void(int64_t, int64_t, LowRank&&, bool)>;
 void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<int32_t> caps,
-             double eps, int nthreads, AcaStats* stats,
-             const std::function<void(int64_t, LowRank&&)>& on_done,
+             double eps, double svd_eps, int nthreads, AcaStats* stats, const AcaDone& on_done,
              const std::vector<int64_t>& probe = {}, const AcaRefine& refine = nullptr);
 // capacity rounding of a pass (multiple of 4, at least 4, at most kmax)
 int64_t aca_round_cap(int64_t c, int64_t kmax);

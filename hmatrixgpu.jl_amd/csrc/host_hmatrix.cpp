@@ -225,10 +225,10 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
             };
         }
         h->device = opt->device;
-        gpu_aca(*h, blocks, caps, eps / 10.0, threads, opt->stats,
-                [&](int64_t b, LowRank&& f) {
-                    h->approx_stats[b].aca_rank = static_cast<int32_t>(f.k);
-                    lr[b] = svd ? svd_recompress(f, eps) : std::move(f);
+        gpu_aca(*h, blocks, caps, eps / 10.0, svd ? eps : -1.0, threads, opt->stats,
+                [&](int64_t b, int64_t aca_k, LowRank&& f, bool recompressed) {
+                    h->approx_stats[b].aca_rank = static_cast<int32_t>(aca_k);
+                    lr[b] = (svd && !recompressed) ? svd_recompress(f, eps) : std::move(f);
                 },
                 probe, refine);
     } else

@@ -38,6 +38,20 @@ int64_t pick_pivot(const double* absval, int64_t n, const std::vector<char>& use
 
 }  // namespace
 
+// Dot product in the fixed order shared with the device recompression: 64
+// lane partials (lane l sums x[l]*y[l], x[l+64]*y[l+64], ... in index order),
+// then p[l] += p[l+off] for off = 32, 16, ..., 1.
+double dot_lanes64(const double* x, const double* y, int64_t n) {
+    double p[64] = {};
+    for (int64_t i = 0; i < n; i++) {
+        const double pr = x[i] * y[i];
+        p[i % 64] = p[i % 64] + pr;
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; l++) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+
 // Sum of squares in the fixed order shared with the device ACA+ kernel: 256
 // lane partials (lane l sums x[l], x[l+256], ... in index order), then a binary
 // tree p[l] += p[l+off] for off = 128, 64, ..., 1.  (Julia's sum in aca.jl:95 is
@@ -56,13 +70,13 @@ double sumsq_lanes(const double* x, int64_t n) {
 
 namespace {
 
-// Householder QR of an m x k column-major panel (m >= k), in place.
+// Householder QR of an m x k column-major panel (m >= k), in place.  The
+// reflector sums use the fixed 64-lane order of dot_lanes64.
 void householder_qr(int64_t m, int64_t k, double* A, double* tau) {
     for (int64_t j = 0; j < k; j++) {
         double* a = A + j * m;
         const double alpha = a[j];
-        double xs = 0.0;
-        for (int64_t i = j + 1; i < m; i++) xs += a[i] * a[i];
+        const double xs = dot_lanes64(a + j + 1, a + j + 1, m - j - 1);
         if (xs == 0.0) {
             tau[j] = 0.0;
             continue;
@@ -75,8 +89,7 @@ void householder_qr(int64_t m, int64_t k, double* A, double* tau) {
         a[j] = beta;
         for (int64_t c = j + 1; c < k; c++) {
             double* b = A + c * m;
-            double w = b[j];
-            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            double w = b[j] + dot_lanes64(a + j + 1, b + j + 1, m - j - 1);
             w = w * tau[j];
             b[j] = b[j] - w;
             for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;
@@ -91,8 +104,7 @@ void apply_q(int64_t m, int64_t k, const double* A, const double* tau, int64_t n
         const double* a = A + j * m;
         for (int64_t c = 0; c < ncols; c++) {
             double* b = B + c * m;
-            double w = b[j];
-            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            double w = b[j] + dot_lanes64(a + j + 1, b + j + 1, m - j - 1);
             w = w * tau[j];
             b[j] = b[j] - w;
             for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;

@@ -32,7 +32,8 @@ class BuildStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("blocks", "chunks", "passes", "launches",
                                          "unconverged_first", "probe_blocks", "slot_elems",
                                          "used_elems")] + \
-               [(n, C.c_double) for n in ("aca_kernel_ms", "aca_wall_s", "build_s", "total_s")]
+               [(n, C.c_double) for n in ("aca_kernel_ms", "recompress_ms", "aca_wall_s", "build_s",
+                                          "total_s")]
 
 
 f32p = C.POINTER(C.c_float)

@@ -267,8 +267,9 @@ end
 mutable struct BuildStats
     blocks::Int64; chunks::Int64; passes::Int64; launches::Int64
     unconverged_first::Int64; probe_blocks::Int64; slot_elems::Int64; used_elems::Int64
-    aca_kernel_ms::Float64; aca_wall_s::Float64; build_s::Float64; total_s::Float64
-    BuildStats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0)
+    aca_kernel_ms::Float64; recompress_ms::Float64; aca_wall_s::Float64; build_s::Float64
+    total_s::Float64
+    BuildStats() = new(0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
 end
 
 """

@@ -144,6 +144,7 @@ typedef struct hmx_build_stats {
     int64_t slot_elems;        /* sum over passes of (m + n) * capacity (allocated terms) */
     int64_t used_elems;        /* sum of (m + n) * ACA+ rank (filled terms) */
     double aca_kernel_ms;      /* GPU ACA+ kernel time */
+    double recompress_ms;      /* GPU recompression (QR + Jacobi SVD) kernel time */
     double aca_wall_s;         /* GPU ACA+ phase incl. copies and recompression */
     double build_s;            /* host build (trees, ACA+, recompression, layout) */
     double total_s;            /* build_s + device upload of the requested storage */
@@ -237,8 +238,10 @@ int hmx_matvec(hmx_hmatrix* h, int32_t storage, const void* X, int64_t ldx, void
 
 /* ---- batched products of independent H-matrices (no reference counterpart:
  *      the reference runs one mul! per H-matrix, mult.jl:159-189) -------------
- * All members must be materialised in `storage` on one device; they must stay
- * alive while the batch exists.  Every stage of the product is one kernel over
+ * All members must be materialised in `storage` on one device.  While the batch
+ * exists its members cannot be released, re-materialised or destroyed (those
+ * calls return HMX_ERR_STATE).  Calls of one batch on different streams must
+ * not overlap (the batch owns one set of scratch arrays).  Every stage of the product is one kernel over
  * all members, and the four stages are replayed as one captured hipGraph per
  * call.  1 <= nrhs <= 16 (the same for all members). */
 typedef struct hmx_batch hmx_batch;

@@ -26,7 +26,7 @@
  *   orc_mul_layout       dense direct sum (orc_dense_matvec) within the H-matrix
  *                        tolerance, and numpy evaluation of the exported layout
  *   orc_sumsq_lanes      exact on integer-valued inputs, and within n*ulp of
- *                        a long-double sum on random inputs
+ *   orc_dot_lanes64      a long-double sum on random inputs
  *   orc_mlp_*            forward: constant and all-active (linear) networks vs
  *                        numpy matrix products; gradient: central finite
  *                        differences of orc_mlp_loss_grad's own loss
@@ -552,12 +552,26 @@ int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const 
 /* ------------------------------------------------------------------------ */
 /* Householder QR (column-major m x k, m >= k).  R is left in the upper
  * triangle, the Householder vectors (implicit unit head) below it. */
+/* Reflector sums of the QR (qr(U), qr(V') of aca.jl:125-126): LAPACK's order is
+ * its own; the order used here is 64 lane partials (lane l: x[l]*y[l],
+ * x[l+64]*y[l+64], ... ascending) folded by p[l] += p[l+off], off = 32 ... 1. */
+double orc_dot_lanes64(const double* x, const double* y, int64_t n) {
+    double p[64];
+    for (int l = 0; l < 64; l++) p[l] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        double pr = x[i] * y[i];
+        p[i % 64] = p[i % 64] + pr;
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; l++) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+
 void orc_qr(int64_t m, int64_t k, double* A, double* tau) {
     for (int64_t j = 0; j < k; j++) {
         double* a = A + j * m;
         double alpha = a[j];
-        double xs = 0.0;
-        for (int64_t i = j + 1; i < m; i++) xs += a[i] * a[i];
+        double xs = orc_dot_lanes64(a + j + 1, a + j + 1, m - j - 1);
         if (xs == 0.0) {
             tau[j] = 0.0;
             continue;
@@ -570,8 +584,7 @@ void orc_qr(int64_t m, int64_t k, double* A, double* tau) {
         a[j] = beta;
         for (int64_t c = j + 1; c < k; c++) {
             double* b = A + c * m;
-            double w = b[j];
-            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            double w = b[j] + orc_dot_lanes64(a + j + 1, b + j + 1, m - j - 1);
             w = w * tau[j];
             b[j] = b[j] - w;
             for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;
@@ -587,8 +600,7 @@ void orc_qr_apply(int64_t m, int64_t k, const double* A, const double* tau, int6
         const double* a = A + j * m;
         for (int64_t c = 0; c < ncols; c++) {
             double* b = B + c * m;
-            double w = b[j];
-            for (int64_t i = j + 1; i < m; i++) w += a[i] * b[i];
+            double w = b[j] + orc_dot_lanes64(a + j + 1, b + j + 1, m - j - 1);
             w = w * tau[j];
             b[j] = b[j] - w;
             for (int64_t i = j + 1; i < m; i++) b[i] = b[i] - a[i] * w;

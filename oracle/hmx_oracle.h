@@ -78,6 +78,8 @@ int64_t orc_aca_plus(const orc_kernel* K, const int64_t* rows, int64_t m, const 
                      int64_t n, double eps, int64_t max_iter, double** U, double** V);
 /* sum of squares in the stop test's fixed 256-lane order */
 double orc_sumsq_lanes(const double* x, int64_t n);
+/* dot product in the QR reflectors' fixed 64-lane order */
+double orc_dot_lanes64(const double* x, const double* y, int64_t n);
 /* SVD_recompress(U, V, eps): returns the truncated rank r; U2 m x r, V2 r x n. */
 int64_t orc_svd_recompress(int64_t m, int64_t n, int64_t k, const double* U, const double* V,
                            double eps, double** U2, double** V2);

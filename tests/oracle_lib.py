@@ -50,6 +50,7 @@ def _load():
         "orc_aca_plus": (C.c_int64, [C.POINTER(Kernel), _i64p, C.c_int64, _i64p, C.c_int64,
                                      C.c_double, C.c_int64, C.POINTER(_f64p), C.POINTER(_f64p)]),
         "orc_sumsq_lanes": (C.c_double, [_f64p, C.c_int64]),
+        "orc_dot_lanes64": (C.c_double, [_f64p, _f64p, C.c_int64]),
         "orc_mlp_nparams": (C.c_int64, []),
         "orc_mlp_forward": (None, [_f64p, _f64p, C.POINTER(C.c_float), C.c_int64, _f64p]),
         "orc_mlp_loss_grad": (None, [_f64p, _f64p, C.POINTER(C.c_float), _f64p, C.c_int64,
@@ -213,6 +214,12 @@ def aca_plus(kernel, rows, cols, eps, max_iter=0):
     U = _take(Up, len(rows) * k).reshape(k, len(rows)).T
     V = _take(Vp, len(cols) * k).reshape(len(cols), k).T
     return U, V
+
+
+def dot_lanes64(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib.orc_dot_lanes64(x.ctypes.data_as(_f64p), y.ctypes.data_as(_f64p), len(x))
 
 
 def sumsq_lanes(x):

@@ -154,3 +154,17 @@ def test_forced_widths_agree(members, cuda):
                 assert np.array_equal(a, b), key   # order changes only the schedule
             else:
                 assert rel(a, b) <= 1e-5, key
+
+
+def test_members_are_pinned_while_the_batch_lives(members, cuda):
+    import hmatrixgpu as H
+    hms, _ = members
+    B = H.HMatrixBatch(hms[:2], storage="f32", nrhs=4)
+    with pytest.raises(H.HMXError, match="live batch"):
+        hms[0].release("f32")
+    with pytest.raises(H.HMXError, match="live batch"):
+        hms[1].materialize("f32")
+    del B
+    import gc
+    gc.collect()
+    hms[0].materialize("f32")   # allowed again once the batch is gone

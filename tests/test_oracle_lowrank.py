@@ -181,3 +181,13 @@ def test_sumsq_lanes_pins():
         p[:off] = p[:off] + p[off:2 * off]
         off //= 2
     assert O.sumsq_lanes(r) == p[0]
+
+
+def test_dot_lanes64_pins():
+    a = np.arange(-300, 500, dtype=np.float64)
+    b = np.arange(800, 0, -1, dtype=np.float64)
+    assert O.dot_lanes64(a, b) == float(np.dot(a.astype(np.int64), b.astype(np.int64)))
+    assert O.dot_lanes64(np.zeros(0), np.zeros(0)) == 0.0
+    r = np.random.default_rng(5).standard_normal((2, 3000))
+    exact = float(np.sum(r[0].astype(np.longdouble) * r[1].astype(np.longdouble)))
+    assert abs(O.dot_lanes64(r[0], r[1]) - exact) <= 3000 * np.finfo(float).eps * np.sum(np.abs(r[0] * r[1]))
