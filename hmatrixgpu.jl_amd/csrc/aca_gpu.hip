@@ -337,24 +337,29 @@ __global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, do
         Z[idx] = i == j ? 1.0 : 0.0;
     }
     __syncthreads();
-    // one-sided Jacobi: cyclic sweeps over p < q
+    // one-sided Jacobi, round-robin sweeps: the k'/2 disjoint pairs of a round
+    // are rotated in parallel, one lane per pair (the host does them in order;
+    // disjoint rotations commute exactly)
     const double tol = 2.220446049250313e-16;
+    const int kp = k + (k & 1);
     for (int sweep = 0; sweep < 80; sweep++) {
         bool rotated = false;
-        for (int p = 0; p + 1 < k; p++) {
-            for (int q = p + 1; q < k; q++) {
+        for (int rnd = 0; rnd + 1 < kp; rnd++) {
+            for (int pi = lane; pi < kp / 2; pi += 64) {
+                const int a = pi == 0 ? 0 : 1 + (pi - 1 + rnd) % (kp - 1);
+                const int b = 1 + (kp - 2 - pi + rnd) % (kp - 1);
+                const int p = min(a, b), q = max(a, b);
+                if (q >= k) continue;
                 double* ap = M + p * k;
                 double* aq = M + q * k;
-                double v = 0.0;
-                if (lane < 3) {  // alpha, beta, gamma: one sequential sum each
-                    const double* x = lane == 1 ? aq : ap;
-                    const double* y = lane == 0 ? ap : aq;
-                    for (int i = 0; i < k; i++) {
-                        const double pr = x[i] * y[i];
-                        v = v + pr;
-                    }
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int i = 0; i < k; i++) {
+                    const double x = ap[i], y = aq[i];
+                    const double xx = x * x, yy = y * y, xy = x * y;
+                    al = al + xx;
+                    be = be + yy;
+                    ga = ga + xy;
                 }
-                const double al = __shfl(v, 0), be = __shfl(v, 1), ga = __shfl(v, 2);
                 if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
                 rotated = true;
                 const double zeta = (be - al) / (2.0 * ga);
@@ -363,7 +368,7 @@ __global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, do
                 const double s = c * t;
                 double* zp = Z + p * k;
                 double* zq = Z + q * k;
-                for (int i = lane; i < k; i += 64) {
+                for (int i = 0; i < k; i++) {
                     const double x = ap[i], y = aq[i];
                     ap[i] = c * x - s * y;
                     aq[i] = s * x + c * y;
@@ -371,10 +376,10 @@ __global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, do
                     zp[i] = c * zx - s * zy;
                     zq[i] = s * zx + c * zy;
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
-        if (!rotated) break;
+        if (!__any(rotated)) break;
     }
     for (int j = lane; j < k; j += 64) {
         double s = 0.0;

@@ -113,15 +113,24 @@ void apply_q(int64_t m, int64_t k, const double* A, const double* tau, int64_t n
 }
 
 // One-sided Jacobi SVD: M (k x k, column-major) = W diag(sig) Z^T.  On return
-// M holds W; sig is descending (stable order).
+// M holds W; sig is descending (stable order).  Round-robin sweeps.
 void jacobi_svd(int64_t k, double* M, double* sig, double* Z) {
     const double tol = 2.220446049250313e-16;
     std::fill(Z, Z + k * k, 0.0);
     for (int64_t i = 0; i < k; i++) Z[i + i * k] = 1.0;
+    // Round-robin (circle) schedule: k' = k rounded up to even players, player 0
+    // fixed, 1 .. k'-1 rotated by the round; the k'/2 pairs of a round touch
+    // disjoint columns, so their order inside the round does not change a bit
+    // (the device rotates them in parallel).
+    const int64_t kp = k + (k & 1);
     for (int sweep = 0; sweep < 80; sweep++) {
         bool rotated = false;
-        for (int64_t p = 0; p + 1 < k; p++) {
-            for (int64_t q = p + 1; q < k; q++) {
+        for (int64_t rnd = 0; rnd + 1 < kp; rnd++) {
+            for (int64_t pi = 0; pi < kp / 2; pi++) {
+                const int64_t a = pi == 0 ? 0 : 1 + (pi - 1 + rnd) % (kp - 1);
+                const int64_t b = 1 + (kp - 2 - pi + rnd) % (kp - 1);
+                const int64_t p = std::min(a, b), q = std::max(a, b);
+                if (q >= k) continue;  // the dummy column of an odd k
                 double* ap = M + p * k;
                 double* aq = M + q * k;
                 double al = 0.0, be = 0.0, ga = 0.0;

@@ -609,16 +609,25 @@ void orc_qr_apply(int64_t m, int64_t k, const double* A, const double* tau, int6
 }
 
 /* One-sided (Hestenes) Jacobi SVD of a k x k column-major M = W diag(sig) Z^T.
- * Cyclic sweeps p<q, rotation skipped when |gamma| <= eps*sqrt(alpha*beta);
- * singular values sorted descending (stable). */
+ * Round-robin sweeps (disjoint pairs per round, each pair p<q), rotation skipped
+ * when |gamma| <= eps*sqrt(alpha*beta); singular values sorted descending
+ * (stable). */
 void orc_jacobi_svd(int64_t k, double* A, double* sig, double* Z) {
     const double tol = 2.220446049250313e-16;
     memset(Z, 0, (size_t)(k * k) * sizeof(double));
     for (int64_t i = 0; i < k; i++) Z[i + i * k] = 1.0;
+    /* sweeps of k' - 1 rounds (k' = k rounded up to even; column k is a dummy
+     * when k is odd): round r pairs the players of the circle schedule,
+     * player 0 fixed and 1 .. k'-1 rotated by r; a round's pairs are disjoint */
+    const int64_t kp = k + (k & 1);
     for (int sweep = 0; sweep < 80; sweep++) {
         int rotated = 0;
-        for (int64_t p = 0; p + 1 < k; p++) {
-            for (int64_t q = p + 1; q < k; q++) {
+        for (int64_t r = 0; r + 1 < kp; r++) {
+            for (int64_t pi = 0; pi < kp / 2; pi++) {
+                int64_t a = pi == 0 ? 0 : 1 + (pi - 1 + r) % (kp - 1);
+                int64_t b = 1 + (kp - 2 - pi + r) % (kp - 1);
+                int64_t p = a < b ? a : b, q = a < b ? b : a;
+                if (q >= k) continue;
                 double* ap = A + p * k;
                 double* aq = A + q * k;
                 double al = 0.0, be = 0.0, ga = 0.0;
