@@ -18,6 +18,9 @@ struct hmx_tree {
 struct hmx_blocktree {
     std::unique_ptr<hmx::BlockTree> b;
 };
+struct hmx_csr {
+    hmx::CsrMatrix c;
+};
 struct hmx_lowrank {
     hmx::LowRank lr;
 };
@@ -250,6 +253,57 @@ int hmx_lowrank_destroy(hmx_lowrank* lr) {
 }
 
 // ---- H-matrix ------------------------------------------------------------------------
+// ---- sparse near field ---------------------------------------------------------------
+int hmx_sparsify(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                 const double* target_xyz, const double* source_xyz, hmx_block_fn fill, void* ctx,
+                 double eta, int32_t nthreads, hmx_csr** out) {
+    return guarded([&] {
+        need(target && source && target->t && source->t && out, "hmx_sparsify: NULL argument");
+        *out = nullptr;
+        const hmx::ClusterTree& T = *target->t;
+        const hmx::ClusterTree& S = *source->t;
+        hmx::EntrySource src;
+        src.kernel_id = kernel_id;
+        if (kernel_id == HMX_KERNEL_CALLBACK) {
+            need(fill != nullptr, "hmx_sparsify: callback kernel without fill function");
+            src.fill = fill;
+            src.ctx = ctx;
+        } else {
+            need(kernel_id == HMX_KERNEL_LAPLACE || kernel_id == HMX_KERNEL_LOG2D,
+                 "hmx_sparsify: unknown kernel id");
+            need(target_xyz && source_xyz, "hmx_sparsify: built-in kernels need coordinates");
+            if (T.dims != 1 || S.dims != 1)
+                throw hmx::Error(HMX_ERR_UNSUPPORTED, "built-in kernels are scalar (dims == 1)");
+            src.xt = target_xyz;
+            src.xs = source_xyz;
+        }
+        *out = new hmx_csr{hmx::sparsify(T, S, src, eta, nthreads)};
+    });
+}
+
+int hmx_csr_sizes(const hmx_csr* c, int64_t* m, int64_t* n, int64_t* nnz) {
+    return guarded([&] {
+        need(c != nullptr, "hmx_csr_sizes: NULL");
+        if (m) *m = c->c.m;
+        if (n) *n = c->c.n;
+        if (nnz) *nnz = static_cast<int64_t>(c->c.col.size());
+    });
+}
+
+int hmx_csr_export(const hmx_csr* c, int64_t* rowptr, int64_t* colidx, double* values) {
+    return guarded([&] {
+        need(c != nullptr, "hmx_csr_export: NULL");
+        if (rowptr) std::copy(c->c.rowptr.begin(), c->c.rowptr.end(), rowptr);
+        if (colidx) std::copy(c->c.col.begin(), c->c.col.end(), colidx);
+        if (values) std::copy(c->c.val.begin(), c->c.val.end(), values);
+    });
+}
+
+int hmx_csr_destroy(hmx_csr* c) {
+    delete c;
+    return HMX_OK;
+}
+
 int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
                       const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
                       void* ctx, double eta, double eps, int32_t svd_recompress,

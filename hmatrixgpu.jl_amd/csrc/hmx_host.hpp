@@ -158,6 +158,16 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
                                        bool svd, int nthreads,
                                        const BuildOptions* opt = nullptr);
 
+// sparsify_hmatrix (hmatrix.jl:234-274): the merged dense blocks as CSR,
+// original index spaces, 0-based, columns sorted within each row.
+struct CsrMatrix {
+    int64_t m = 0, n = 0;
+    std::vector<int64_t> rowptr, col;
+    std::vector<double> val;
+};
+CsrMatrix sparsify(const ClusterTree& T, const ClusterTree& S, const EntrySource& src, double eta,
+                   int nthreads);
+
 // Simple fixed-size worker pool: run f(i) for i in [0,n) on `threads` threads.
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f);
 int default_threads();

@@ -309,4 +309,89 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
     return h;
 }
 
+// ---- sparsify_hmatrix (hmatrix.jl:234-274) ------------------------------------------
+// Every entry of the merged dense blocks of the block tree, as a CSR matrix in
+// the original index spaces (0-based, columns sorted within a row).  Rows are
+// owned by exactly one target leaf, so leaves are filled in parallel.
+CsrMatrix sparsify(const ClusterTree& T, const ClusterTree& S, const EntrySource& src, double eta,
+                   int nthreads) {
+    auto bt = build_block_tree(T, S, eta);  // hmatrix.jl:236
+    merge_dense(*bt);                       // :237
+    std::vector<std::pair<int64_t, int64_t>> direct, approx;
+    traverse(*bt, &direct, &approx);        // :239
+    bt.reset();
+    CsrMatrix out;
+    out.m = static_cast<int64_t>(T.index_map.size());
+    out.n = static_cast<int64_t>(S.index_map.size());
+    // dense blocks covering each target leaf (a block's rows are a union of leaves)
+    const int64_t nl = static_cast<int64_t>(T.leaves.size());
+    std::vector<int64_t> leaf_start(nl);
+    for (int64_t l = 0; l < nl; l++) leaf_start[l] = T.nodes[T.leaves[l]].start;
+    std::vector<std::vector<int32_t>> blocks_of(nl);
+    for (size_t d = 0; d < direct.size(); d++) {
+        const ClusterNode& a = T.nodes[direct[d].first];
+        const int64_t l0 = std::lower_bound(leaf_start.begin(), leaf_start.end(), a.start) -
+                           leaf_start.begin();
+        for (int64_t l = l0; l < nl && leaf_start[l] < a.end; l++)
+            blocks_of[l].push_back(static_cast<int32_t>(d));
+    }
+    auto leaf_rows = [&](int64_t l) {  // permuted positions [r0, r1) of leaf l (node
+        const ClusterNode& L = T.nodes[T.leaves[l]];  // ranges are in index-map units)
+        return std::make_pair(L.start - 1, L.end - 1);
+    };
+    std::vector<int64_t> row_nnz(static_cast<size_t>(out.m), 0);
+    for (int64_t l = 0; l < nl; l++) {
+        int64_t w = 0;
+        for (int32_t d : blocks_of[l]) {
+            const ClusterNode& b = S.nodes[direct[d].second];
+            w += b.end - b.start;
+        }
+        auto [r0, r1] = leaf_rows(l);
+        for (int64_t p = r0; p < r1; p++) row_nnz[T.index_map[p] - 1] = w;
+    }
+    out.rowptr.assign(static_cast<size_t>(out.m + 1), 0);
+    for (int64_t r = 0; r < out.m; r++) out.rowptr[r + 1] = out.rowptr[r] + row_nnz[r];
+    const int64_t nnz = out.rowptr[out.m];
+    out.col.resize(static_cast<size_t>(nnz));
+    out.val.resize(static_cast<size_t>(nnz));
+    const int threads = src.thread_safe() ? (nthreads > 0 ? nthreads : default_threads()) : 1;
+    parallel_for(nl, threads, [&](int64_t l) {
+        auto [r0, r1] = leaf_rows(l);
+        const int64_t nr = r1 - r0;
+        std::vector<int64_t> rows(static_cast<size_t>(nr)), cols;
+        for (int64_t i = 0; i < nr; i++) rows[i] = T.index_map[r0 + i] - 1;
+        std::vector<int64_t> cursor(static_cast<size_t>(nr));
+        for (int64_t i = 0; i < nr; i++) cursor[i] = out.rowptr[rows[i]];
+        std::vector<double> blk;
+        for (int32_t d : blocks_of[l]) {
+            const ClusterNode& b = S.nodes[direct[d].second];
+            const int64_t c0 = b.start - 1, nc = b.end - b.start;
+            cols.resize(static_cast<size_t>(nc));
+            for (int64_t j = 0; j < nc; j++) cols[j] = S.index_map[c0 + j] - 1;
+            blk.resize(static_cast<size_t>(nr * nc));
+            src.block(rows.data(), nr, cols.data(), nc, blk.data());  // K[target_ids, source_ids]
+            for (int64_t i = 0; i < nr; i++)
+                for (int64_t j = 0; j < nc; j++) {
+                    out.col[cursor[i]] = cols[j];
+                    out.val[cursor[i]] = blk[i + j * nr];
+                    cursor[i]++;
+                }
+        }
+        // sort each row by column (the blocks' columns are disjoint ranges)
+        std::vector<std::pair<int64_t, double>> tmp;
+        for (int64_t i = 0; i < nr; i++) {
+            const int64_t a = out.rowptr[rows[i]], e = out.rowptr[rows[i] + 1];
+            tmp.resize(static_cast<size_t>(e - a));
+            for (int64_t q = a; q < e; q++) tmp[q - a] = {out.col[q], out.val[q]};
+            std::sort(tmp.begin(), tmp.end(),
+                      [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (int64_t q = a; q < e; q++) {
+                out.col[q] = tmp[q - a].first;
+                out.val[q] = tmp[q - a].second;
+            }
+        }
+    });
+    return out;
+}
+
 }  // namespace hmx

@@ -69,6 +69,11 @@ SIGNATURES = {
                                        C.POINTER(BuildOptions), C.POINTER(BuildStats),
                                        C.POINTER(vp)]),
     "hmx_hmatrix_block_features": (C.c_int, [vp, i64p, f32p, i32p, i32p, i32p]),
+    "hmx_sparsify": (C.c_int, [vp, vp, C.c_int32, f64p, f64p, BLOCK_FN, vp, C.c_double, C.c_int32,
+                               C.POINTER(vp)]),
+    "hmx_csr_sizes": (C.c_int, [vp, i64p, i64p, i64p]),
+    "hmx_csr_export": (C.c_int, [vp, i64p, i64p, f64p]),
+    "hmx_csr_destroy": (C.c_int, [vp]),
     "hmx_rankpred_create": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(vp)]),
     "hmx_rankpred_nparams": (C.c_int, [i64p]),
     "hmx_rankpred_train": (C.c_int, [vp, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,

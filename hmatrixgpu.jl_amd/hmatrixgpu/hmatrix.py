@@ -340,6 +340,38 @@ def _torch():
     return torch
 
 
+def sparsify_hmatrix(K, X, Y, eta=1.5, index_map_using_cpu=True, nthreads=0):
+    """sparsify_hmatrix(K, X, Y; eta) (hmatrix.jl:234-274): every entry of the
+    merged dense blocks as a scipy.sparse CSR matrix in the original index
+    spaces (the reference returns a SparseMatrixCSC with the same entries)."""
+    import scipy.sparse as sp
+    errors = []
+    kid = getattr(K, "kernel_id", None)
+    h = L.vp()
+    if kid is not None:
+        xt = np.ascontiguousarray(K.X.T)
+        ys = np.ascontiguousarray(K.Y.T)
+        st = L.lib.hmx_sparsify(X.handle, Y.handle, kid, L.ptr_f64(xt), L.ptr_f64(ys),
+                                L.BLOCK_FN(), None, float(eta), int(nthreads), C.byref(h))
+    else:
+        cb = _block_callback(K, errors)
+        st = L.lib.hmx_sparsify(X.handle, Y.handle, L.HMX_KERNEL_CALLBACK, None, None, cb, None,
+                                float(eta), int(nthreads), C.byref(h))
+    if errors:
+        raise errors[0]
+    L.check(st)
+    try:
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(L.lib.hmx_csr_sizes(h, C.byref(m), C.byref(n), C.byref(nnz)))
+        rp = np.zeros(m.value + 1, np.int64)
+        ci = np.zeros(max(nnz.value, 1), np.int64)
+        va = np.zeros(max(nnz.value, 1), np.float64)
+        L.check(L.lib.hmx_csr_export(h, L.ptr_i64(rp), L.ptr_i64(ci), L.ptr_f64(va)))
+    finally:
+        L.lib.hmx_csr_destroy(h)
+    return sp.csr_matrix((va[:nnz.value], ci[:nnz.value], rp), shape=(m.value, n.value))
+
+
 def _ld(t):
     """Leading dimension of a vector or column-major matrix tensor.  A single
     column's stride is meaningless (torch normalises it), so it is the row count."""

@@ -7,12 +7,15 @@
 # product runs in gfx950 HIP kernels on device arrays (see ext/AMDGPUExt.jl).
 module HMatrixGPU
 
+using SparseArrays
+
 using LinearAlgebra
 import LinearAlgebra: mul!
 import Base: *
 
 export ClusterTree, BlockTree, ACA_plus, HMatrix, info, set_groupsize, set_backend, @using_gpu
 export RankPredictor, train!, predict, block_features, HMatrixBatch, hmx_batch_mul!, BuildStats
+export sparsify_hmatrix
 
 const libhmx = get(ENV, "HMX_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib", "libhmx.so"))
 
@@ -456,6 +459,38 @@ function hmx_batch_mul!(B::HMatrixBatch, Y::Vector{<:Ptr}, X::Vector{<:Ptr}; str
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Cvoid}),
                 B.handle, Ptr{Cvoid}.(X), C_NULL, Ptr{Cvoid}.(Y), C_NULL, stream))
     return nothing
+end
+
+"""
+    sparsify_hmatrix(K, X, Y; eta=1.5, index_map_using_cpu=true)
+
+Every entry of the merged dense blocks as a SparseMatrixCSC (hmatrix.jl:234-274).
+"""
+function sparsify_hmatrix(K::AbstractMatrix, X::ClusterTree, Y::ClusterTree; eta=1.5,
+                          index_map_using_cpu=true)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    kid = _kernel_id(K)
+    cb = @cfunction(_block_cb, Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}))
+    GC.@preserve K begin
+        xt = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.X)
+        xs = kid == HMX_KERNEL_CALLBACK ? C_NULL : pointer(K.Y)
+        check(ccall((:hmx_sparsify, libhmx), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}, Any,
+                     Float64, Int32, Ptr{Ptr{Cvoid}}),
+                    X.handle, Y.handle, kid, xt, xs, cb, K, Float64(eta), Int32(0), out))
+    end
+    m, n, nnz = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:hmx_csr_sizes, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                out[], m, n, nnz))
+    rp, ci, va = Vector{Int64}(undef, m[] + 1), Vector{Int64}(undef, nnz[]), Vector{Float64}(undef, nnz[])
+    check(ccall((:hmx_csr_export, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                out[], rp, ci, va))
+    ccall((:hmx_csr_destroy, libhmx), Cint, (Ptr{Cvoid},), out[])
+    I = Vector{Int64}(undef, nnz[])
+    for r in 1:m[], q in (rp[r] + 1):rp[r + 1]
+        I[q] = r
+    end
+    return sparse(I, ci .+ 1, va, m[], n[])
 end
 
 # ---- backend registry (HMatrixGPU.jl:14-109) -----------------------------------------

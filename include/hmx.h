@@ -119,6 +119,21 @@ int hmx_hmatrix_build(const hmx_tree* target, const hmx_tree* source, int32_t ke
                       const double* target_xyz, const double* source_xyz, hmx_block_fn fill,
                       void* ctx, double eta, double eps, int32_t svd_recompress,
                       int32_t storage_mask, int32_t device, int32_t nthreads, hmx_hmatrix** out);
+/* ---- sparse near field: sparsify_hmatrix(K, X, Y; eta) (hmatrix.jl:234-274) --
+ * Builds the block tree of (target, source) at eta, merges dense blocks
+ * (block.jl:98-112) and returns every entry of the dense blocks as a sparse
+ * matrix in the ORIGINAL index spaces: CSR, 0-based, columns sorted within a
+ * row (the reference returns a SparseMatrixCSC with the same entries).
+ * Kernel arguments as for hmx_hmatrix_build; nthreads <= 0: default. */
+typedef struct hmx_csr hmx_csr;
+int hmx_sparsify(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
+                 const double* target_xyz, const double* source_xyz, hmx_block_fn fill, void* ctx,
+                 double eta, int32_t nthreads, hmx_csr** out);
+int hmx_csr_sizes(const hmx_csr* c, int64_t* m, int64_t* n, int64_t* nnz);
+/* rowptr: m + 1 entries, colidx / values: nnz entries; any may be NULL */
+int hmx_csr_export(const hmx_csr* c, int64_t* rowptr, int64_t* colidx, double* values);
+int hmx_csr_destroy(hmx_csr* c);
+
 /* ---- build options beyond the reference's keywords ------------------------- */
 typedef struct hmx_rankpred hmx_rankpred;
 typedef struct hmx_build_options {
