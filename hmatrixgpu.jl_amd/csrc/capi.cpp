@@ -253,6 +253,21 @@ int hmx_lowrank_destroy(hmx_lowrank* lr) {
 }
 
 // ---- H-matrix ------------------------------------------------------------------------
+// Built-in kernels: the scalar ones need dims == 1 on both trees, the dipole
+// tensor dims == 3.
+static void check_builtin(int32_t kernel_id, const hmx::ClusterTree& T,
+                          const hmx::ClusterTree& S) {
+    if (kernel_id == HMX_KERNEL_LAPLACE || kernel_id == HMX_KERNEL_LOG2D) {
+        if (T.dims != 1 || S.dims != 1)
+            throw hmx::Error(HMX_ERR_UNSUPPORTED, "scalar built-in kernels need dims == 1");
+    } else if (kernel_id == HMX_KERNEL_DIPOLE) {
+        if (T.dims != 3 || S.dims != 3)
+            throw hmx::Error(HMX_ERR_UNSUPPORTED, "the dipole tensor kernel needs dims == 3");
+    } else {
+        throw hmx::Error(HMX_ERR_ARG, "unknown kernel id");
+    }
+}
+
 // ---- sparse near field ---------------------------------------------------------------
 int hmx_sparsify(const hmx_tree* target, const hmx_tree* source, int32_t kernel_id,
                  const double* target_xyz, const double* source_xyz, hmx_block_fn fill, void* ctx,
@@ -269,11 +284,8 @@ int hmx_sparsify(const hmx_tree* target, const hmx_tree* source, int32_t kernel_
             src.fill = fill;
             src.ctx = ctx;
         } else {
-            need(kernel_id == HMX_KERNEL_LAPLACE || kernel_id == HMX_KERNEL_LOG2D,
-                 "hmx_sparsify: unknown kernel id");
             need(target_xyz && source_xyz, "hmx_sparsify: built-in kernels need coordinates");
-            if (T.dims != 1 || S.dims != 1)
-                throw hmx::Error(HMX_ERR_UNSUPPORTED, "built-in kernels are scalar (dims == 1)");
+            check_builtin(kernel_id, T, S);
             src.xt = target_xyz;
             src.xs = source_xyz;
         }
@@ -353,11 +365,8 @@ int hmx_hmatrix_build_ex(const hmx_tree* target, const hmx_tree* source, int32_t
             src.fill = fill;
             src.ctx = ctx;
         } else {
-            need(kernel_id == HMX_KERNEL_LAPLACE || kernel_id == HMX_KERNEL_LOG2D,
-                 "hmx_hmatrix_build: unknown kernel id");
             need(target_xyz && source_xyz, "hmx_hmatrix_build: built-in kernels need coordinates");
-            if (T.dims != 1 || S.dims != 1)
-                throw hmx::Error(HMX_ERR_UNSUPPORTED, "built-in kernels are scalar (dims == 1)");
+            check_builtin(kernel_id, T, S);
             xt.assign(target_xyz, target_xyz + 3 * T.npts);
             xs.assign(source_xyz, source_xyz + 3 * S.npts);
             src.xt = xt.data();

@@ -146,8 +146,11 @@ void device_build(HMatrix& h, int storage) {
     if (h.device < 0 || h.device >= ndev) throw Error(HMX_ERR_ARG, "invalid device ordinal");
     if (h.m >= (int64_t(1) << 31) || h.n + h.R + 64 >= (int64_t(1) << 31))
         throw Error(HMX_ERR_UNSUPPORTED, "matrix too large for 32-bit device indices");
-    if (h.src.kernel_id != HMX_KERNEL_CALLBACK && (h.target_dims != 1 || h.source_dims != 1))
-        throw Error(HMX_ERR_UNSUPPORTED, "built-in kernels need dims == 1");
+    if ((h.src.kernel_id == HMX_KERNEL_LAPLACE || h.src.kernel_id == HMX_KERNEL_LOG2D) &&
+        (h.target_dims != 1 || h.source_dims != 1))
+        throw Error(HMX_ERR_UNSUPPORTED, "scalar built-in kernels need dims == 1");
+    if (h.src.kernel_id == HMX_KERNEL_DIPOLE && (h.target_dims != 3 || h.source_dims != 3))
+        throw Error(HMX_ERR_UNSUPPORTED, "the dipole tensor kernel needs dims == 3");
     DeviceGuard guard(h.device);
     auto op = std::make_unique<DeviceOp>();
     op->storage = storage;

@@ -61,7 +61,34 @@ void parallel_for(int64_t n, int threads, const std::function<void(int64_t)>& f)
 }
 
 // ---- entries ---------------------------------------------------------------
+namespace {
+// dipolar_tensor_xx / _xy of examples/hmatrix_vector.jl:12-28, same operation
+// order (R is summed from the arguments as passed)
+double dip_xx(double x, double y, double z) {
+    const double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -(2 * x * x - y * y - z * z) / (R * R * std::sqrt(R)) / (4 * 3.141592653589793);
+}
+double dip_xy(double x, double y, double z) {
+    const double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -3 * x * y / (R * R * std::sqrt(R)) / (4 * 3.141592653589793);
+}
+// component (a, b) of dipolar_tensor(x, y, z) (hmatrix_vector.jl:30-51)
+double dipole(int a, int b, double x, double y, double z) {
+    if (a == b) return a == 0 ? dip_xx(x, y, z) : a == 1 ? dip_xx(y, x, z) : dip_xx(z, y, x);
+    const int s = a + b;  // 1: xy, 2: xz, 3: yz
+    return s == 1 ? dip_xy(x, y, z) : s == 2 ? dip_xy(x, z, y) : dip_xy(y, z, x);
+}
+}  // namespace
+
 double EntrySource::entry(int64_t i, int64_t j) const {
+    if (kernel_id == HMX_KERNEL_DIPOLE) {  // degrees of freedom 3p + a, 3q + b
+        const double* a = xt + 3 * (i / 3);
+        const double* b = xs + 3 * (j / 3);
+        return dipole(static_cast<int>(i % 3), static_cast<int>(j % 3), a[0] - b[0], a[1] - b[1],
+                      a[2] - b[2]);
+    }
     const double* a = xt + 3 * i;
     const double* b = xs + 3 * j;
     if (kernel_id == HMX_KERNEL_LAPLACE) {

@@ -597,7 +597,7 @@ __global__ __launch_bounds__(256) void k_gather(const InstDesc one, const InstDe
 
 // ---- build-time panel generation ------------------------------------------------
 // Column descriptors of the leaf panels: kind 0 = built-in kernel entry
-// (base = 0-based source point), 1 = U payload, 2 = dense payload (callback
+// (base = 0-based original source index), 1 = U payload, 2 = dense payload (callback
 // kernels), 3 = padding.  Buffer element (row r) = base + r * stride.
 struct ColDesc {
     int64_t base;
@@ -622,6 +622,39 @@ __device__ inline double kernel_entry(int kid, const double* a, const double* b)
     s += r2 * r2;
     const double d = sqrt(s);
     return d < 1e-12 ? 1.0 : (-0.5 / 3.141592653589793) * log(d);
+}
+
+// dipolar_tensor_xx / _xy of examples/hmatrix_vector.jl:12-28 (same operation
+// order as the host and the reference)
+__device__ inline double dip_xx(double x, double y, double z) {
+#pragma clang fp contract(off)
+    const double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -(2 * x * x - y * y - z * z) / (R * R * sqrt(R)) / (4 * 3.141592653589793);
+}
+__device__ inline double dip_xy(double x, double y, double z) {
+#pragma clang fp contract(off)
+    const double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -3 * x * y / (R * R * sqrt(R)) / (4 * 3.141592653589793);
+}
+
+// Entry of a built-in kernel between original target index i and source index
+// j: points for the scalar kernels, degrees of freedom 3p + a for the dipole
+// tensor.
+__device__ inline double kernel_entry_ij(int kid, const double* xt, const double* xs, int64_t i,
+                                         int64_t j) {
+#pragma clang fp contract(off)
+    if (kid == HMX_KERNEL_DIPOLE) {
+        const double* a = xt + 3 * (i / 3);
+        const double* b = xs + 3 * (j / 3);
+        const double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+        const int ca = static_cast<int>(i % 3), cb = static_cast<int>(j % 3);
+        if (ca == cb) return ca == 0 ? dip_xx(x, y, z) : ca == 1 ? dip_xx(y, x, z) : dip_xx(z, y, x);
+        const int s = ca + cb;
+        return s == 1 ? dip_xy(x, y, z) : s == 2 ? dip_xy(x, z, y) : dip_xy(y, z, x);
+    }
+    return kernel_entry(kid, xt + 3 * i, xs + 3 * j);
 }
 
 // Element (row r, column k) of an item, at tile position q (16-byte chunks).
@@ -654,14 +687,14 @@ __global__ __launch_bounds__(256) void k_gen_leaf(const Item* __restrict__ items
         int kt, r, k0;
         chunk_coords<TS>(q, it.rows, kt, r, k0);
         const int64_t trow = it.out_off + r;
-        const double* a = xt ? xt + 3 * static_cast<int64_t>(tmap0[trow]) : nullptr;
+        const int64_t ti = tmap0[trow];  // original target index
         TS vals[EPL];
 #pragma unroll
         for (int e = 0; e < EPL; e++) {
             const ColDesc cd = cols[it.col_off + k0 + e];
             double v = 0.0;
             if (cd.kind == 0)
-                v = kernel_entry(kid, a, xs + 3 * cd.base);
+                v = kernel_entry_ij(kid, xt, xs, ti, cd.base);
             else if (cd.kind == 1)
                 v = Ubuf[cd.base + static_cast<int64_t>(r) * cd.stride];
             else if (cd.kind == 2)

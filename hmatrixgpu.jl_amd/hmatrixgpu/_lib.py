@@ -12,7 +12,7 @@ LIB_PATH = os.environ.get("HMX_LIBRARY", os.path.join(PKG_ROOT, "lib", "libhmx.s
 
 HMX_OK = 0
 HMX_F64, HMX_F32, HMX_BF16 = 0, 1, 2
-HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D = 0, 1, 2
+HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D, HMX_KERNEL_DIPOLE = 0, 1, 2, 3
 
 i64p = C.POINTER(C.c_int64)
 f64p = C.POINTER(C.c_double)

@@ -61,6 +61,49 @@ class LaplaceKernel(_PointKernel):
             return np.where(d == 0.0, 0.0, 1.0 / d)
 
 
+class DipoleKernel(_PointKernel):
+    """The dipolar (demagnetisation) tensor of examples/hmatrix_vector.jl:12-51:
+    K[3p+a, 3q+b] = N_ab(x_p - y_q), N = -(3 r r^T - |r|^2 I)/(4 pi |r|^5), 0 at
+    r = 0.  Use with ClusterTree(..., dims=3) on both sides."""
+
+    kernel_id = L.HMX_KERNEL_DIPOLE
+
+    def __init__(self, target_points, source_points=None):
+        super().__init__(target_points, source_points)
+        self.shape = (3 * self.X.shape[1], 3 * self.Y.shape[1])
+
+    def __getitem__(self, idx):
+        rows, cols = idx  # index arrays (any shape broadcasting to rows x cols)
+        rows = np.asarray(rows).ravel()
+        cols = np.asarray(cols).ravel()
+        out = np.empty((len(rows), len(cols)))
+        for a, i in enumerate(rows.tolist()):
+            for b, j in enumerate(cols.tolist()):
+                r = self.X[:, i // 3] - self.Y[:, j // 3]
+                out[a, b] = _dipole(i % 3, j % 3, float(r[0]), float(r[1]), float(r[2]))
+        return out
+
+
+def _dipole_xx(x, y, z):
+    R = x * x + y * y + z * z
+    if R == 0:
+        return 0.0
+    return -(2 * x * x - y * y - z * z) / (R * R * math.sqrt(R)) / (4 * math.pi)
+
+
+def _dipole_xy(x, y, z):
+    R = x * x + y * y + z * z
+    if R == 0:
+        return 0.0
+    return -3 * x * y / (R * R * math.sqrt(R)) / (4 * math.pi)
+
+
+def _dipole(a, b, x, y, z):
+    if a == b:
+        return (_dipole_xx(x, y, z), _dipole_xx(y, x, z), _dipole_xx(z, y, x))[a]
+    return {1: _dipole_xy(x, y, z), 2: _dipole_xy(x, z, y), 3: _dipole_xy(y, z, x)}[a + b]
+
+
 class Log2DKernel(_PointKernel):
     """K[i, j] = -log(d)/(2 pi), 1.0 where d < 1e-12 (test_hmatrix.jl:17-20)."""
 

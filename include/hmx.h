@@ -42,7 +42,11 @@ enum hmx_dtype { HMX_F64 = 0, HMX_F32 = 1, HMX_BF16 = 2 };
 enum hmx_kernel_id {
     HMX_KERNEL_CALLBACK = 0, /* entries come from an hmx_block_fn               */
     HMX_KERNEL_LAPLACE = 1,  /* 1/|x-y|; 0 where points coincide (the diagonal)  */
-    HMX_KERNEL_LOG2D = 2     /* -log(d)/(2*pi); 1.0 where d < 1e-12              */
+    HMX_KERNEL_LOG2D = 2,    /* -log(d)/(2*pi); 1.0 where d < 1e-12              */
+    HMX_KERNEL_DIPOLE = 3    /* dipolar (demagnetisation) tensor, dims = 3 on both
+                                trees: entry (3p+a, 3q+b) = N_ab(x_p - y_q),
+                                N = -(3 r r^T - |r|^2 I) / (4 pi |r|^5), 0 at r = 0
+                                (examples/hmatrix_vector.jl:12-51)               */
 };
 
 typedef struct hmx_tree hmx_tree;

@@ -84,8 +84,33 @@ double orc_norm3(const double v[3]) {
     return maxabs * sqrt(s);
 }
 
+/* dipolar_tensor_xx / _xy, examples/hmatrix_vector.jl:12-28 */
+static double dip_xx(double x, double y, double z) {
+    double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -(2 * x * x - y * y - z * z) / (R * R * sqrt(R)) / (4 * 3.141592653589793);
+}
+static double dip_xy(double x, double y, double z) {
+    double R = x * x + y * y + z * z;
+    if (R == 0) return 0.0;
+    return -3 * x * y / (R * R * sqrt(R)) / (4 * 3.141592653589793);
+}
+/* dipolar_tensor(x, y, z)[a, b], hmatrix_vector.jl:30-51:
+ * [xx xy xz; xy yy yz; xz yz zz], yy = xx(y,x,z), zz = xx(z,y,x),
+ * xz = xy(x,z,y), yz = xy(y,z,x) */
+double orc_dipole(int a, int b, double x, double y, double z) {
+    if (a == b) return a == 0 ? dip_xx(x, y, z) : a == 1 ? dip_xx(y, x, z) : dip_xx(z, y, x);
+    int s = a + b;
+    return s == 1 ? dip_xy(x, y, z) : s == 2 ? dip_xy(x, z, y) : dip_xy(y, z, x);
+}
+
 double orc_kernel_entry(const orc_kernel* K, int64_t i, int64_t j) {
     switch (K->kind) {
+    case ORC_K_DIPOLE: {
+        const double* a = K->xt + 3 * (i / 3);
+        const double* b = K->xs + 3 * (j / 3);
+        return orc_dipole((int)(i % 3), (int)(j % 3), a[0] - b[0], a[1] - b[1], a[2] - b[2]);
+    }
     case ORC_K_LAPLACE: {
         const double* a = K->xt + 3 * i;
         const double* b = K->xs + 3 * j;

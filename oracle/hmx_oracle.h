@@ -22,7 +22,8 @@ enum {
     ORC_K_LAPLACE = 1, /* 1/|x-y|, 0 where the points coincide (diagonal)   */
     ORC_K_LOG2D = 2,   /* -log(d)/(2 pi), 1.0 where d < 1e-12 (test_hmatrix.jl:17-20) */
     ORC_K_MATRIX = 3,  /* explicit column-major matrix                        */
-    ORC_K_VEC3 = 4     /* R[j]/r^2, m x 3n, 1.0 where r < 1e-12 (test_hmatrix_vector.jl:14-19) */
+    ORC_K_VEC3 = 4,    /* R[j]/r^2, m x 3n, 1.0 where r < 1e-12 (test_hmatrix_vector.jl:14-19) */
+    ORC_K_DIPOLE = 5   /* dipolar tensor, 3m x 3n (examples/hmatrix_vector.jl:12-51)  */
 };
 
 typedef struct orc_kernel {
@@ -36,6 +37,8 @@ typedef struct orc_kernel {
 
 /* Entry K[i,j] with 0-based indices. */
 double orc_kernel_entry(const orc_kernel* K, int64_t i, int64_t j);
+/* Component (a, b) of the dipolar tensor at r = (x, y, z). */
+double orc_dipole(int a, int b, double x, double y, double z);
 
 /* Julia LinearAlgebra.generic_norm2 of a 3-vector (the path tree.jl:86-87 and
  * block.jl:63 take for length-3 arrays). */

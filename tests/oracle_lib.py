@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
-K_LAPLACE, K_LOG2D, K_MATRIX, K_VEC3 = 1, 2, 3, 4
+K_LAPLACE, K_LOG2D, K_MATRIX, K_VEC3, K_DIPOLE = 1, 2, 3, 4, 5
 
 _i64p = C.POINTER(C.c_int64)
 _f64p = C.POINTER(C.c_double)
@@ -50,6 +50,7 @@ def _load():
         "orc_aca_plus": (C.c_int64, [C.POINTER(Kernel), _i64p, C.c_int64, _i64p, C.c_int64,
                                      C.c_double, C.c_int64, C.POINTER(_f64p), C.POINTER(_f64p)]),
         "orc_sumsq_lanes": (C.c_double, [_f64p, C.c_int64]),
+        "orc_dipole": (C.c_double, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
         "orc_dot_lanes64": (C.c_double, [_f64p, _f64p, C.c_int64]),
         "orc_mlp_nparams": (C.c_int64, []),
         "orc_mlp_forward": (None, [_f64p, _f64p, C.POINTER(C.c_float), C.c_int64, _f64p]),
