@@ -15,12 +15,13 @@ import Base: *
 
 export ClusterTree, BlockTree, ACA_plus, HMatrix, info, set_groupsize, set_backend, @using_gpu
 export RankPredictor, train!, predict, block_features, HMatrixBatch, hmx_batch_mul!, BuildStats
-export sparsify_hmatrix
+export sparsify_hmatrix, LaplaceKernel, DipoleKernel
 
 const libhmx = get(ENV, "HMX_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib", "libhmx.so"))
 
 const HMX_F64, HMX_F32, HMX_BF16 = Int32(0), Int32(1), Int32(2)
 const HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D = Int32(0), Int32(1), Int32(2)
+const HMX_KERNEL_DIPOLE = Int32(3)
 
 function check(st)
     st == 0 && return nothing
@@ -215,6 +216,34 @@ function Base.getindex(K::LaplaceKernel, i::Int, j::Int)
     return d == 0 ? 0.0 : 1.0 / d
 end
 _kernel_id(::LaplaceKernel) = HMX_KERNEL_LAPLACE
+
+"""
+The dipolar (demagnetisation) tensor of examples/hmatrix_vector.jl, entry
+(3p+a, 3q+b) = N_ab(x_p - y_q); build with ClusterTree(...; dims=3) on both
+sides.  Evaluated natively on host and GPU.
+"""
+struct DipoleKernel <: AbstractMatrix{Float64}
+    X::Matrix{Float64}
+    Y::Matrix{Float64}
+end
+Base.size(K::DipoleKernel) = (3 * size(K.X, 2), 3 * size(K.Y, 2))
+function _dip_xx(x, y, z)
+    R = x * x + y * y + z * z
+    return R == 0 ? 0.0 : -(2 * x * x - y * y - z * z) / (R * R * sqrt(R)) / (4 * pi)
+end
+function _dip_xy(x, y, z)
+    R = x * x + y * y + z * z
+    return R == 0 ? 0.0 : -3 * x * y / (R * R * sqrt(R)) / (4 * pi)
+end
+function Base.getindex(K::DipoleKernel, i::Int, j::Int)
+    p, a = divrem(i - 1, 3)
+    q, b = divrem(j - 1, 3)
+    x, y, z = K.X[1, p + 1] - K.Y[1, q + 1], K.X[2, p + 1] - K.Y[2, q + 1], K.X[3, p + 1] - K.Y[3, q + 1]
+    a == b && return (_dip_xx(x, y, z), _dip_xx(y, x, z), _dip_xx(z, y, x))[a + 1]
+    s = a + b
+    return s == 1 ? _dip_xy(x, y, z) : s == 2 ? _dip_xy(x, z, y) : _dip_xy(y, z, x)
+end
+_kernel_id(::DipoleKernel) = HMX_KERNEL_DIPOLE
 _kernel_id(::AbstractMatrix) = HMX_KERNEL_CALLBACK
 
 # Any AbstractMatrix: entries through the getindex contract K[rows, cols]
