@@ -605,6 +605,27 @@ int hmx_hmatrix_export_block_ranges(const hmx_hmatrix* hm, int64_t* dense_ranges
     });
 }
 
+int hmx_hmatrix_save(hmx_hmatrix* hm, const char* path, int32_t flags) {
+    return guarded([&] {
+        need(hm && hm->h && path, "hmx_hmatrix_save: NULL argument");
+        need((flags & ~HMX_SAVE_DENSE) == 0, "hmx_hmatrix_save: unknown flags");
+        hmx::save_hmatrix(*hm->h, path, (flags & HMX_SAVE_DENSE) != 0);
+    });
+}
+
+int hmx_hmatrix_load(const char* path, int32_t storage_mask, int32_t device, hmx_hmatrix** out) {
+    return guarded([&] {
+        need(path && out, "hmx_hmatrix_load: NULL argument");
+        need(storage_mask >= 0 && storage_mask < 8, "hmx_hmatrix_load: bad storage_mask");
+        *out = nullptr;
+        auto h = hmx::load_hmatrix(path);
+        h->device = device;
+        for (int s = 0; s < 3; s++)
+            if (storage_mask & (1 << s)) hmx::device_build(*h, s);
+        *out = new hmx_hmatrix{std::move(h)};
+    });
+}
+
 int hmx_hmatrix_info(const hmx_hmatrix* hm, double* out) {
     return guarded([&] {
         need(hm && hm->h && out, "hmx_hmatrix_info: NULL");

@@ -158,6 +158,12 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
                                        bool svd, int nthreads,
                                        const BuildOptions* opt = nullptr);
 
+// On-disk H-matrix (host_io.cpp).  save() writes the dense payload when it is
+// present or with_dense is set (it is always present for callback kernels);
+// load() validates the file and returns a host operator with no device storage.
+void save_hmatrix(HMatrix& h, const char* path, bool with_dense);
+std::unique_ptr<HMatrix> load_hmatrix(const char* path);
+
 // sparsify_hmatrix (hmatrix.jl:234-274): the merged dense blocks as CSR,
 // original index spaces, 0-based, columns sorted within each row.
 struct CsrMatrix {

@@ -1,0 +1,367 @@
+// Serialized H-matrix (the built host operator: permutations, block lists,
+// low-rank factors, entry source, predictor statistics) so that a build can be
+// cached on disk and re-materialised on a device without ACA+.  The reference
+// has no file format; the contents are those of HMatrix (hmatrix.jl:46-57) plus
+// what the device planner needs (target leaf starts, source-cluster ids).
+//
+// File layout, little-endian, every section 8-byte aligned:
+//   FileHeader (256 bytes)
+//   tmap[m], smap[n]                      int64, 1-based
+//   dense[nd]   {r0, r1, c0, c1, off}     int64 x 5
+//   lowrank[nl] {r0, r1, c0, c1, k, uoff, voff, vx0, snode}  int64 x 9
+//   U[nU], V[nV]                          double
+//   D[dlen] (flag kHasDense)              double
+//   xt[3 * m / target_dims], xs[...]      double (flag kHasCoords)
+//   target_leaf_start[nleaf + 1]          int64, 0-based with end sentinel
+//   approx_stats[na] {feat[6] f32, aca_rank, rank, cap i32}, zero-padded to 8
+// The header carries a hash of itself and one of the payload; load() checks
+// both, then checks every index against the sizes it addresses before an
+// H-matrix is handed out.
+#include "hmx_host.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+
+namespace hmx {
+namespace {
+
+constexpr char kMagic[8] = {'H', 'M', 'X', 'H', 'M', 'A', 'T', '\0'};
+constexpr uint32_t kVersion = 1;
+constexpr uint32_t kEndian = 0x01020304u;
+constexpr int32_t kHasDense = 1, kHasCoords = 2;
+enum Count { kTmap, kSmap, kDense, kLowRank, kU, kV, kD, kXt, kXs, kLeafStart, kApprox, kCounts };
+constexpr int kDenseWords = 5, kLowRankWords = 9, kApproxBytes = 36;
+
+struct FileHeader {
+    char magic[8];
+    uint32_t version, endian;
+    int64_t m, n, R, dlen;
+    int32_t kernel_id, flags;
+    int64_t target_dims, source_dims;
+    int64_t count[kCounts];
+    double build_seconds;
+    uint64_t payload_hash;
+    uint64_t header_hash;  // over the bytes before this field
+    uint8_t pad[256 - 8 - 8 - 32 - 8 - 16 - 8 * kCounts - 8 - 8 - 8];
+};
+static_assert(sizeof(FileHeader) == 256, "header size");
+static_assert(sizeof(HMatrix::ApproxStat) == kApproxBytes, "approx stat size");
+
+// Four independent multiply-rotate lanes over 8-byte words (word w goes to
+// lane w mod 4), folded at the end: fast enough to hash tens of GB at memory
+// speed, and any changed word changes the result with high probability.
+struct Hash64 {
+    static constexpr uint64_t P1 = 0x9E3779B185EBCA87ull, P2 = 0xC2B2AE3D27D4EB4Full;
+    uint64_t lane[4] = {P1, P2, ~P1, ~P2};
+    uint64_t words = 0;
+    static uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+    void update(const void* p, size_t bytes) {  // bytes % 8 == 0
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < bytes; i += 8, words++) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            uint64_t& l = lane[words & 3];
+            l = rotl(l ^ (w * P2), 31) * P1;
+        }
+    }
+    uint64_t digest() const {
+        uint64_t h = words * P1;
+        for (int i = 0; i < 4; i++) h = rotl(h ^ (lane[i] * P2), 27) * P1 + P2;
+        return h ^ (h >> 29);
+    }
+};
+
+uint64_t header_hash(const FileHeader& h) {
+    Hash64 x;
+    x.update(&h, offsetof(FileHeader, header_hash));
+    return x.digest();
+}
+
+size_t pad8(size_t b) { return (b + 7) & ~size_t(7); }
+
+struct Writer {
+    FILE* f;
+    Hash64 hash;
+    std::string path;
+    void raw(const void* p, size_t bytes) {
+        if (bytes && std::fwrite(p, 1, bytes, f) != bytes)
+            throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: write failed: " + path);
+    }
+    // payload section: hashed, zero-padded to 8 bytes
+    void section(const void* p, size_t bytes) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        const size_t chunk = size_t(1) << 26;
+        size_t full = bytes & ~size_t(7);
+        for (size_t o = 0; o < full; o += chunk) {
+            const size_t c = std::min(chunk, full - o);
+            hash.update(b + o, c);
+            raw(b + o, c);
+        }
+        if (bytes != full) {
+            uint8_t tail[8] = {0};
+            std::memcpy(tail, b + full, bytes - full);
+            hash.update(tail, 8);
+            raw(tail, 8);
+        }
+    }
+};
+
+struct Reader {
+    FILE* f;
+    Hash64 hash;
+    std::string path;
+    void raw(void* p, size_t bytes) {
+        if (bytes && std::fread(p, 1, bytes, f) != bytes)
+            throw Error(HMX_ERR_ARG, "hmx_hmatrix_load: truncated file: " + path);
+    }
+    void section(void* p, size_t bytes) {
+        uint8_t* b = static_cast<uint8_t*>(p);
+        const size_t chunk = size_t(1) << 26;
+        size_t full = bytes & ~size_t(7);
+        for (size_t o = 0; o < full; o += chunk) {
+            const size_t c = std::min(chunk, full - o);
+            raw(b + o, c);
+            hash.update(b + o, c);
+        }
+        if (bytes != full) {
+            uint8_t tail[8];
+            raw(tail, 8);
+            hash.update(tail, 8);
+            std::memcpy(b + full, tail, bytes - full);
+        }
+    }
+};
+
+struct FileCloser {
+    void operator()(FILE* f) const {
+        if (f) std::fclose(f);
+    }
+};
+
+void bad(const std::string& what) {
+    throw Error(HMX_ERR_ARG, "hmx_hmatrix_load: inconsistent file (" + what + ")");
+}
+
+// every entry of map in [1, len] and each value once
+void check_permutation(const std::vector<int64_t>& map, const char* name) {
+    std::vector<uint8_t> seen(map.size(), 0);
+    for (int64_t v : map) {
+        if (v < 1 || v > static_cast<int64_t>(map.size()) || seen[v - 1]) bad(name);
+        seen[v - 1] = 1;
+    }
+}
+
+// a * b <= limit for non-negative a, b, without overflow
+bool fits(int64_t a, int64_t b, int64_t limit) {
+    return a >= 0 && b >= 0 && (a == 0 || b <= limit / a);
+}
+
+}  // namespace
+
+void save_hmatrix(HMatrix& h, const char* path, bool with_dense) {
+    if (!path) throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: NULL path");
+    const bool builtin = h.src.kernel_id != HMX_KERNEL_CALLBACK;
+    if (with_dense) h.ensure_dense_payload();
+    const bool has_dense = h.D_ready;
+    if (!builtin && !has_dense) throw Error(HMX_ERR_STATE, "hmx_hmatrix_save: dense payload missing");
+    FileHeader hd{};
+    std::memcpy(hd.magic, kMagic, 8);
+    hd.version = kVersion;
+    hd.endian = kEndian;
+    hd.m = h.m;
+    hd.n = h.n;
+    hd.R = h.R;
+    hd.dlen = h.dlen;
+    hd.kernel_id = h.src.kernel_id;
+    hd.flags = (has_dense ? kHasDense : 0) | (builtin ? kHasCoords : 0);
+    hd.target_dims = h.target_dims;
+    hd.source_dims = h.source_dims;
+    hd.count[kTmap] = static_cast<int64_t>(h.tmap.size());
+    hd.count[kSmap] = static_cast<int64_t>(h.smap.size());
+    hd.count[kDense] = static_cast<int64_t>(h.dense.size());
+    hd.count[kLowRank] = static_cast<int64_t>(h.lowrank.size());
+    hd.count[kU] = static_cast<int64_t>(h.U.size());
+    hd.count[kV] = static_cast<int64_t>(h.V.size());
+    hd.count[kD] = has_dense ? h.dlen : 0;
+    hd.count[kXt] = builtin ? static_cast<int64_t>(h.xt_copy.size()) : 0;
+    hd.count[kXs] = builtin ? static_cast<int64_t>(h.xs_copy.size()) : 0;
+    hd.count[kLeafStart] = static_cast<int64_t>(h.target_leaf_start.size());
+    hd.count[kApprox] = static_cast<int64_t>(h.approx_stats.size());
+    hd.build_seconds = h.build_seconds;
+
+    const std::string final_path(path), tmp = final_path + ".tmp";
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(tmp.c_str(), "wb"));
+    if (!f) throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: cannot open " + tmp);
+    Writer w{f.get(), {}, tmp};
+    w.raw(&hd, sizeof hd);  // rewritten with the hashes at the end
+    w.section(h.tmap.data(), h.tmap.size() * 8);
+    w.section(h.smap.data(), h.smap.size() * 8);
+    {
+        std::vector<int64_t> t(h.dense.size() * kDenseWords);
+        for (size_t b = 0; b < h.dense.size(); b++) {
+            const DenseBlock& d = h.dense[b];
+            const int64_t e[kDenseWords] = {d.r0, d.r1, d.c0, d.c1, d.off};
+            std::copy(e, e + kDenseWords, t.begin() + b * kDenseWords);
+        }
+        w.section(t.data(), t.size() * 8);
+    }
+    {
+        std::vector<int64_t> t(h.lowrank.size() * kLowRankWords);
+        for (size_t b = 0; b < h.lowrank.size(); b++) {
+            const LowRankBlock& l = h.lowrank[b];
+            const int64_t e[kLowRankWords] = {l.r0, l.r1, l.c0, l.c1, l.k, l.uoff, l.voff, l.vx0, l.snode};
+            std::copy(e, e + kLowRankWords, t.begin() + b * kLowRankWords);
+        }
+        w.section(t.data(), t.size() * 8);
+    }
+    w.section(h.U.data(), h.U.size() * 8);
+    w.section(h.V.data(), h.V.size() * 8);
+    if (has_dense) w.section(h.D.data(), static_cast<size_t>(h.dlen) * 8);
+    if (builtin) {
+        w.section(h.xt_copy.data(), h.xt_copy.size() * 8);
+        w.section(h.xs_copy.data(), h.xs_copy.size() * 8);
+    }
+    w.section(h.target_leaf_start.data(), h.target_leaf_start.size() * 8);
+    w.section(h.approx_stats.data(), h.approx_stats.size() * kApproxBytes);
+    hd.payload_hash = w.hash.digest();
+    hd.header_hash = header_hash(hd);
+    if (std::fseek(f.get(), 0, SEEK_SET) != 0) throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: seek failed");
+    w.raw(&hd, sizeof hd);
+    if (std::fclose(f.release()) != 0) throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: close failed");
+    if (std::rename(tmp.c_str(), final_path.c_str()) != 0)
+        throw Error(HMX_ERR_ARG, "hmx_hmatrix_save: cannot rename to " + final_path);
+}
+
+std::unique_ptr<HMatrix> load_hmatrix(const char* path) {
+    if (!path) throw Error(HMX_ERR_ARG, "hmx_hmatrix_load: NULL path");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));
+    if (!f) throw Error(HMX_ERR_ARG, std::string("hmx_hmatrix_load: cannot open ") + path);
+    FileHeader hd;
+    Reader r{f.get(), {}, path};
+    r.raw(&hd, sizeof hd);
+    if (std::memcmp(hd.magic, kMagic, 8) != 0) bad("not an H-matrix file");
+    if (hd.endian != kEndian) bad("byte order");
+    if (hd.version != kVersion)
+        throw Error(HMX_ERR_UNSUPPORTED, "hmx_hmatrix_load: file version " + std::to_string(hd.version));
+    if (hd.header_hash != header_hash(hd)) bad("header checksum");
+
+    // sizes: everything the sections hold must fit the file
+    std::fseek(f.get(), 0, SEEK_END);
+    const int64_t file_bytes = std::ftell(f.get());
+    std::fseek(f.get(), sizeof hd, SEEK_SET);
+    const int64_t words_max = file_bytes / 8;
+    for (int c = 0; c < kCounts; c++)
+        if (hd.count[c] < 0 || hd.count[c] > words_max) bad("section size");
+    const bool has_dense = hd.flags & kHasDense, has_coords = hd.flags & kHasCoords;
+    const bool builtin = hd.kernel_id != HMX_KERNEL_CALLBACK;
+    if (hd.m <= 0 || hd.n <= 0 || hd.count[kTmap] != hd.m || hd.count[kSmap] != hd.n) bad("m, n");
+    if (hd.target_dims <= 0 || hd.source_dims <= 0 || hd.m % hd.target_dims || hd.n % hd.source_dims)
+        bad("dims");
+    if (hd.R < 0 || hd.dlen < 0 || (has_dense && hd.count[kD] != hd.dlen) || (!has_dense && hd.count[kD]))
+        bad("payload lengths");
+    if (builtin != has_coords) bad("entry source");
+    if (!builtin && !has_dense) bad("callback kernel without dense payload");
+    if (builtin) {
+        const bool scalar = hd.kernel_id == HMX_KERNEL_LAPLACE || hd.kernel_id == HMX_KERNEL_LOG2D;
+        if (scalar && (hd.target_dims != 1 || hd.source_dims != 1)) bad("scalar kernel with dims != 1");
+        if (hd.kernel_id == HMX_KERNEL_DIPOLE && (hd.target_dims != 3 || hd.source_dims != 3))
+            bad("dipole kernel with dims != 3");
+        if (!scalar && hd.kernel_id != HMX_KERNEL_DIPOLE) bad("kernel id");
+        if (hd.count[kXt] != 3 * (hd.m / hd.target_dims) || hd.count[kXs] != 3 * (hd.n / hd.source_dims))
+            bad("coordinates");
+    }
+    if (hd.count[kLeafStart] < 2) bad("target leaves");
+    int64_t expect = sizeof hd;
+    const int64_t words = hd.count[kTmap] + hd.count[kSmap] + kDenseWords * hd.count[kDense] +
+                          kLowRankWords * hd.count[kLowRank] + hd.count[kU] + hd.count[kV] +
+                          hd.count[kD] + hd.count[kXt] + hd.count[kXs] + hd.count[kLeafStart];
+    expect += 8 * words + static_cast<int64_t>(pad8(static_cast<size_t>(hd.count[kApprox]) * kApproxBytes));
+    if (expect != file_bytes) bad("file length");
+
+    auto h = std::make_unique<HMatrix>();
+    h->m = hd.m;
+    h->n = hd.n;
+    h->R = hd.R;
+    h->dlen = hd.dlen;
+    h->target_dims = hd.target_dims;
+    h->source_dims = hd.source_dims;
+    h->build_seconds = hd.build_seconds;
+    h->tmap.resize(hd.m);
+    h->smap.resize(hd.n);
+    r.section(h->tmap.data(), hd.m * 8);
+    r.section(h->smap.data(), hd.n * 8);
+    {
+        std::vector<int64_t> t(hd.count[kDense] * kDenseWords);
+        r.section(t.data(), t.size() * 8);
+        h->dense.resize(hd.count[kDense]);
+        for (size_t b = 0; b < h->dense.size(); b++) {
+            const int64_t* e = t.data() + b * kDenseWords;
+            h->dense[b] = DenseBlock{e[0], e[1], e[2], e[3], e[4]};
+        }
+    }
+    {
+        std::vector<int64_t> t(hd.count[kLowRank] * kLowRankWords);
+        r.section(t.data(), t.size() * 8);
+        h->lowrank.resize(hd.count[kLowRank]);
+        for (size_t b = 0; b < h->lowrank.size(); b++) {
+            const int64_t* e = t.data() + b * kLowRankWords;
+            h->lowrank[b] = LowRankBlock{e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8]};
+        }
+    }
+    h->U.resize(hd.count[kU]);
+    h->V.resize(hd.count[kV]);
+    r.section(h->U.data(), h->U.size() * 8);
+    r.section(h->V.data(), h->V.size() * 8);
+    if (has_dense) {
+        h->D.resize(hd.dlen);
+        r.section(h->D.data(), h->D.size() * 8);
+    }
+    if (has_coords) {
+        h->xt_copy.resize(hd.count[kXt]);
+        h->xs_copy.resize(hd.count[kXs]);
+        r.section(h->xt_copy.data(), h->xt_copy.size() * 8);
+        r.section(h->xs_copy.data(), h->xs_copy.size() * 8);
+    }
+    h->target_leaf_start.resize(hd.count[kLeafStart]);
+    r.section(h->target_leaf_start.data(), h->target_leaf_start.size() * 8);
+    h->approx_stats.resize(hd.count[kApprox]);
+    r.section(h->approx_stats.data(), h->approx_stats.size() * kApproxBytes);
+    if (r.hash.digest() != hd.payload_hash) bad("payload checksum");
+
+    // structure: every index the device planner and the host export follow
+    check_permutation(h->tmap, "tmap");
+    check_permutation(h->smap, "smap");
+    for (const DenseBlock& d : h->dense) {
+        if (d.r0 < 1 || d.r1 < d.r0 || d.r1 > h->m || d.c0 < 1 || d.c1 < d.c0 || d.c1 > h->n) bad("dense block range");
+        const int64_t rows = d.r1 - d.r0 + 1, cols = d.c1 - d.c0 + 1;
+        if (!fits(rows, cols, h->dlen) || d.off < 0 || d.off > h->dlen - rows * cols)
+            bad("dense block offset");
+    }
+    std::map<int64_t, std::pair<int64_t, int64_t>> cols_of_source;
+    for (const LowRankBlock& l : h->lowrank) {
+        if (l.r0 < 1 || l.r1 < l.r0 || l.r1 > h->m || l.c0 < 1 || l.c1 < l.c0 || l.c1 > h->n)
+            bad("low-rank block range");
+        const int64_t rows = l.r1 - l.r0 + 1, cols = l.c1 - l.c0 + 1;
+        const int64_t nu = static_cast<int64_t>(h->U.size()), nv = static_cast<int64_t>(h->V.size());
+        if (l.k < 0 || !fits(rows, l.k, nu) || !fits(cols, l.k, nv)) bad("low-rank block rank");
+        if (l.uoff < 0 || l.uoff > nu - rows * l.k || l.voff < 0 || l.voff > nv - cols * l.k)
+            bad("low-rank block offsets");
+        if (l.vx0 < 0 || l.vx0 > h->R - l.k) bad("low-rank block Vx rows");
+        auto it = cols_of_source.emplace(l.snode, std::make_pair(l.c0, l.c1)).first;
+        if (it->second != std::make_pair(l.c0, l.c1)) bad("source cluster columns");
+    }
+    const auto& ls = h->target_leaf_start;
+    if (ls.front() != 0 || ls.back() != h->m) bad("target leaf bounds");
+    for (size_t i = 1; i < ls.size(); i++)
+        if (ls[i] <= ls[i - 1]) bad("target leaf order");
+
+    h->src.kernel_id = hd.kernel_id;
+    h->src.xt = h->xt_copy.empty() ? nullptr : h->xt_copy.data();
+    h->src.xs = h->xs_copy.empty() ? nullptr : h->xs_copy.data();
+    h->D_ready = has_dense;
+    return h;
+}
+
+}  // namespace hmx

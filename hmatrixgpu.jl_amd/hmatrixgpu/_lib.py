@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("HMX_LIBRARY", os.path.join(PKG_ROOT, "lib", "libhmx.s
 HMX_OK = 0
 HMX_F64, HMX_F32, HMX_BF16 = 0, 1, 2
 HMX_KERNEL_CALLBACK, HMX_KERNEL_LAPLACE, HMX_KERNEL_LOG2D, HMX_KERNEL_DIPOLE = 0, 1, 2, 3
+HMX_SAVE_DENSE = 1
 
 i64p = C.POINTER(C.c_int64)
 f64p = C.POINTER(C.c_double)
@@ -89,6 +90,8 @@ SIGNATURES = {
                                                       f64p, f64p]),
     "hmx_hmatrix_export_block_ranges": (C.c_int, [vp, i64p, i64p]),
     "hmx_hmatrix_info": (C.c_int, [vp, f64p]),
+    "hmx_hmatrix_save": (C.c_int, [vp, C.c_char_p, C.c_int32]),
+    "hmx_hmatrix_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(vp)]),
     "hmx_hmatrix_destroy": (C.c_int, [vp]),
     "hmx_matvec": (C.c_int, [vp, C.c_int32, vp, C.c_int64, vp, C.c_int64, C.c_int32, C.c_int32,
                              vp]),

@@ -19,6 +19,7 @@ product.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -186,6 +187,10 @@ class HMatrix:
         if errors:
             raise errors[0]
         L.check(st)
+        self._attach(h)
+        self.build_stats = {name: getattr(stats, name) for name, _ in L.BuildStats._fields_}
+
+    def _attach(self, h):
         self._h = h
         s = np.zeros(8, np.int64)
         L.check(L.lib.hmx_hmatrix_sizes(h, L.ptr_i64(s)))
@@ -193,7 +198,34 @@ class HMatrix:
          self._vlen) = [int(v) for v in s]
         self._layout = None
         self._scratch = {}
-        self.build_stats = {name: getattr(stats, name) for name, _ in L.BuildStats._fields_}
+
+    # ---- on-disk operator (hmx_hmatrix_save / hmx_hmatrix_load) -----------------
+    def save(self, path, dense=False):
+        """Write the host operator to `path` (no reference counterpart).  With
+        dense=True a built-in kernel's dense payload is written too; a callback
+        kernel's always is."""
+        L.check(L.lib.hmx_hmatrix_save(self._h, os.fsencode(path),
+                                       L.HMX_SAVE_DENSE if dense else 0))
+
+    @classmethod
+    def load(cls, path, storage=("f64",), device=0):
+        """An H-matrix read from a file written by save(), materialised in
+        `storage` on `device`.  The kernel, trees and build parameters are not in
+        the file (K, X, Y, eta, eps are None)."""
+        mask = 0
+        for st in ([storage] if isinstance(storage, str) else storage):
+            mask |= 1 << STORAGE[st.lower()]
+        h = L.vp()
+        L.check(L.lib.hmx_hmatrix_load(os.fsencode(path), mask, int(device), C.byref(h)))
+        self = object.__new__(cls)
+        self.K = self.X = self.Y = None
+        self.eta = self.eps = None
+        self.flatten = cls is not HMatrixCPU
+        self.storage_mask = mask
+        self.device = int(device)
+        self._attach(h)
+        self.build_stats = {}
+        return self
 
     def block_features(self):
         """Per admissible block (approx-list order): features (n x 6: log diam_s,

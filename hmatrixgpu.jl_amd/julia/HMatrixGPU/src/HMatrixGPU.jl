@@ -15,7 +15,7 @@ import Base: *
 
 export ClusterTree, BlockTree, ACA_plus, HMatrix, info, set_groupsize, set_backend, @using_gpu
 export RankPredictor, train!, predict, block_features, HMatrixBatch, hmx_batch_mul!, BuildStats
-export sparsify_hmatrix, LaplaceKernel, DipoleKernel
+export sparsify_hmatrix, LaplaceKernel, DipoleKernel, save_hmatrix, load_hmatrix
 
 const libhmx = get(ENV, "HMX_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib", "libhmx.so"))
 
@@ -262,7 +262,7 @@ end
 
 # ---- H-matrix (hmatrix.jl:18-57, 72-148) ---------------------------------------------
 mutable struct HMatrix{T<:AbstractFloat}
-    K::AbstractMatrix
+    K::Union{Nothing,AbstractMatrix}  # nothing for an H-matrix read by load_hmatrix
     handle::Ptr{Cvoid}
     storage::Int32
     m::Int
@@ -427,6 +427,33 @@ function info(H::HMatrix)  # hmatrix.jl:289-330 plus byte counts
                 "leaves" => Int(o[1]), "admissible_leaves" => Int(o[2]), "full_leaves" => Int(o[3]),
                 "min_rank" => Int(o[4]), "max_rank" => Int(o[5]), "min_dense_size" => Int(o[6]),
                 "max_dense_size" => Int(o[7]), "compression_ratio" => o[8])
+end
+
+"""
+    save_hmatrix(path, H; dense=false)
+    load_hmatrix(path; storage=Float64, device=0) -> HMatrix
+
+On-disk H-matrix (hmx_hmatrix_save / hmx_hmatrix_load; no reference
+counterpart): index maps, block lists, FP64 factors and the entry source
+(coordinates of a built-in kernel, the dense payload of a user matrix).  The
+loaded H-matrix is materialised in `storage` on `device`; its `K` is `nothing`.
+"""
+function save_hmatrix(path::AbstractString, H::HMatrix; dense::Bool=false)
+    check(ccall((:hmx_hmatrix_save, libhmx), Cint, (Ptr{Cvoid}, Cstring, Int32),
+                getfield(H, :handle), path, dense ? Int32(1) : Int32(0)))
+    return path
+end
+function load_hmatrix(path::AbstractString; storage=Float64, device=0)
+    st = storage === Float64 ? HMX_F64 : storage === Float32 ? HMX_F32 : HMX_BF16
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:hmx_hmatrix_load, libhmx), Cint, (Cstring, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                path, Int32(1) << st, Int32(device), out))
+    s = Vector{Int64}(undef, 8)
+    check(ccall((:hmx_hmatrix_sizes, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}), out[], s))
+    T = storage === Float64 ? Float64 : Float32
+    H = HMatrix{T}(nothing, out[], st, s[1], s[2])
+    finalizer(x -> ccall((:hmx_hmatrix_destroy, libhmx), Cint, (Ptr{Cvoid},), x.handle), H)
+    return H
 end
 
 """

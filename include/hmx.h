@@ -241,6 +241,21 @@ int hmx_hmatrix_export_block_ranges(const hmx_hmatrix* h, int64_t* dense_ranges,
  * [8] nnz_D, [9] nnz_U, [10] nnz_V, [11] device panel bytes F64, [12] F32,
  * [13] BF16, [14] device index/plan bytes (all storages), [15] build seconds */
 int hmx_hmatrix_info(const hmx_hmatrix* h, double* out);
+/* ---- on-disk H-matrix (no reference counterpart: the reference rebuilds
+ *      every HMatrix from its kernel, hmatrix.jl:72-148) ----------------------
+ * hmx_hmatrix_save writes the host operator (index maps, block lists, FP64
+ * factors, the entry source: coordinates for built-in kernels, the dense
+ * payload for callback kernels; the per-block rank statistics) to `path`
+ * through a temporary file renamed at the end.  flags: HMX_SAVE_DENSE also
+ * writes the dense payload of a built-in kernel (evaluating it on the host if
+ * needed).  Device storage is not saved.
+ * hmx_hmatrix_load reads such a file, checks its checksums and every index
+ * against the sizes it addresses (HMX_ERR_ARG on any mismatch), and returns an
+ * H-matrix equal to the saved one, materialised in `storage_mask` on `device`
+ * like hmx_hmatrix_build.  Files are little-endian, version 1. */
+enum { HMX_SAVE_DENSE = 1 };
+int hmx_hmatrix_save(hmx_hmatrix* h, const char* path, int32_t flags);
+int hmx_hmatrix_load(const char* path, int32_t storage_mask, int32_t device, hmx_hmatrix** out);
 int hmx_hmatrix_destroy(hmx_hmatrix* h);
 
 /* ---- product: replaces mul!(result, H::HMatrix{T}, x) (mult.jl:159-189) ----

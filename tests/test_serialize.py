@@ -1,0 +1,237 @@
+"""On-disk H-matrix (hmx_hmatrix_save / hmx_hmatrix_load), CPU side.
+
+A loaded operator must equal the saved one: the exported reference layout
+(hmatrix.jl:95-147: index maps, block tables, FP64 payloads), info() and the
+per-block rank statistics, bit for bit.  Damaged files are refused with
+HMX_ERR_ARG: a flipped payload byte by the checksum, and indices that point
+outside the arrays they address (written with a valid checksum) by the
+structural checks that run before any device planning.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import hmatrixgpu as H
+from helpers import uniform_points
+
+HEADER = 256
+M64 = (1 << 64) - 1
+P1, P2 = 0x9E3779B185EBCA87, 0xC2B2AE3D27D4EB4F
+
+
+def _rotl(x, r):
+    return ((x << r) | (x >> (64 - r))) & M64
+
+
+def _hash(data):
+    """The file's checksum: four multiply-rotate lanes over 8-byte words."""
+    lanes = [P1, P2, ~P1 & M64, ~P2 & M64]
+    words = np.frombuffer(data, dtype="<u8")
+    for i, w in enumerate(words.tolist()):
+        lanes[i & 3] = (_rotl(lanes[i & 3] ^ ((w * P2) & M64), 31) * P1) & M64
+    h = (len(words) * P1) & M64
+    for lane in lanes:
+        h = (_rotl(h ^ ((lane * P2) & M64), 27) * P1 + P2) & M64
+    return h ^ (h >> 29)
+
+
+def _reseal(raw):
+    """Recompute both checksums of a (modified) file image."""
+    raw = bytearray(raw)
+    struct.pack_into("<Q", raw, 168, _hash(bytes(raw[HEADER:])))
+    struct.pack_into("<Q", raw, 176, _hash(bytes(raw[:176])))
+    return bytes(raw)
+
+
+def _layouts_equal(a, b):
+    la, lb = a.layout(True), b.layout(True)
+    assert la.keys() == lb.keys()
+    for k in la:
+        assert np.array_equal(np.asarray(la[k]), np.asarray(lb[k])), k
+
+
+def _stats_equal(a, b):
+    fa, fb = a.block_features(), b.block_features()
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+
+
+@pytest.fixture(scope="module")
+def laplace():
+    pts = uniform_points(1200, 41)
+    T = H.ClusterTree(pts, max_points_per_leaf=32)
+    return H.HMatrix(H.LaplaceKernel(pts), T, T, eta=1.5, eps=1e-6, storage=())
+
+
+def test_builtin_round_trip(laplace, tmp_path):
+    p = tmp_path / "lap.hmx"
+    laplace.save(p)
+    g = H.HMatrix.load(p, storage=())
+    assert g.shape == laplace.shape and g.R == laplace.R
+    assert (g.n_dense, g.n_lowrank) == (laplace.n_dense, laplace.n_lowrank)
+    _layouts_equal(laplace, g)  # dense payload re-generated from the saved coordinates
+    _stats_equal(laplace, g)
+    ia, ib = laplace.info(), g.info()
+    for k in ("leaves", "admissible_leaves", "full_leaves", "min_rank", "max_rank",
+              "min_dense_size", "max_dense_size", "compression_ratio", "nnz_D", "nnz_U", "nnz_V",
+              "build_seconds"):
+        assert ia[k] == ib[k], k
+    assert not os.path.exists(str(p) + ".tmp")
+
+
+def test_dense_flag_stores_the_dense_payload(tmp_path):
+    pts = uniform_points(800, 46)
+    T = H.ClusterTree(pts, max_points_per_leaf=32)
+    h = H.HMatrix(H.LaplaceKernel(pts), T, T, eta=1.5, eps=1e-6, storage=())
+    a, b, c = tmp_path / "a.hmx", tmp_path / "b.hmx", tmp_path / "c.hmx"
+    h.save(a)                 # built-in kernel: coordinates only
+    h.save(b, dense=True)     # evaluates and writes the dense payload
+    h.save(c)                 # the payload is now present and written as well
+    assert os.path.getsize(b) - os.path.getsize(a) == 8 * h._dlen
+    assert open(b, "rb").read() == open(c, "rb").read()
+    _layouts_equal(h, H.HMatrix.load(b, storage=()))
+
+
+def test_callback_round_trip(tmp_path):
+    pts = uniform_points(700, 42)
+    T = H.ClusterTree(pts, max_points_per_leaf=24)
+    Kb = H.LaplaceKernel(pts)
+
+    class Callback:
+        shape = Kb.shape
+
+        def __getitem__(self, idx):
+            return Kb[idx]
+
+    h = H.HMatrix(Callback(), T, T, eta=1.0, eps=1e-5, storage=())
+    p = tmp_path / "cb.hmx"
+    h.save(p)
+    g = H.HMatrix.load(p, storage=())
+    _layouts_equal(h, g)
+
+
+def test_dipole_round_trip(tmp_path):
+    pts = uniform_points(250, 43)
+    T = H.ClusterTree(pts, max_points_per_leaf=16, dims=3)
+    h = H.HMatrix(H.DipoleKernel(pts), T, T, eta=1.0, eps=1e-6, storage=())
+    p = tmp_path / "dip.hmx"
+    h.save(p)
+    g = H.HMatrix.load(p, storage=())
+    assert g.shape == (750, 750)
+    _layouts_equal(h, g)
+
+
+def test_rectangular_round_trip(tmp_path):
+    xs = uniform_points(900, 44)
+    xt = uniform_points(500, 45) + np.array([[2.0], [0.0], [0.0]])
+    Tt = H.ClusterTree(xt, max_points_per_leaf=32)
+    Ts = H.ClusterTree(xs, max_points_per_leaf=32)
+    h = H.HMatrix(H.LaplaceKernel(xt, xs), Tt, Ts, eta=1.5, eps=1e-6, storage=())
+    p = tmp_path / "rect.hmx"
+    h.save(p)
+    g = H.HMatrix.load(p, storage=())
+    assert g.shape == (500, 900)
+    _layouts_equal(h, g)
+
+
+def test_hash_restatement_matches_the_file(laplace, tmp_path):
+    p = tmp_path / "h.hmx"
+    laplace.save(p)
+    raw = open(p, "rb").read()
+    assert _reseal(raw) == raw
+
+
+def _expect_refused(raw, tmp_path, match):
+    p = tmp_path / "bad.hmx"
+    open(p, "wb").write(raw)
+    with pytest.raises(H.HMXError, match=match) as e:
+        H.HMatrix.load(p, storage=())
+    assert e.value.code == 1  # HMX_ERR_ARG
+
+
+def _sections(raw):
+    """Byte offsets of the index sections, from the header's counts."""
+    m, n = struct.unpack_from("<qq", raw, 16)
+    counts = struct.unpack_from("<11q", raw, 72)
+    off = {"tmap": HEADER}
+    off["smap"] = off["tmap"] + 8 * counts[0]
+    off["dense"] = off["smap"] + 8 * counts[1]
+    off["lowrank"] = off["dense"] + 40 * counts[2]
+    off["U"] = off["lowrank"] + 72 * counts[3]
+    return m, n, counts, off
+
+
+def test_damaged_files_are_refused(laplace, tmp_path):
+    p = tmp_path / "ok.hmx"
+    laplace.save(p)
+    raw = open(p, "rb").read()
+    m, n, counts, off = _sections(raw)
+    # a flipped payload bit
+    bad = bytearray(raw)
+    bad[off["U"] + 8 * 5] ^= 0x10
+    _expect_refused(bytes(bad), tmp_path, "payload checksum")
+    # a header field changed without resealing
+    bad = bytearray(raw)
+    struct.pack_into("<q", bad, 16, m + 1)
+    _expect_refused(bytes(bad), tmp_path, "header checksum")
+    # truncated / not a file of this kind / unknown version
+    _expect_refused(raw[:-8], tmp_path, "inconsistent file")
+    _expect_refused(b"NOTHMX\0\0" + raw[8:], tmp_path, "not an H-matrix file")
+    bad = bytearray(raw)
+    struct.pack_into("<I", bad, 8, 2)
+    p2 = tmp_path / "v2.hmx"
+    open(p2, "wb").write(_reseal(bytes(bad)))
+    with pytest.raises(H.HMXError, match="version 2"):
+        H.HMatrix.load(p2, storage=())
+
+
+@pytest.mark.parametrize("field", ["tmap_dup", "dense_rows", "dense_off", "lr_rank", "lr_voff",
+                                   "lr_vx0", "lr_cols"])
+def test_out_of_range_indices_are_refused_with_a_valid_checksum(laplace, tmp_path, field):
+    p = tmp_path / "ok.hmx"
+    laplace.save(p)
+    raw = bytearray(open(p, "rb").read())
+    m, n, counts, off = _sections(raw)
+    d0 = off["dense"]           # first dense block {r0, r1, c0, c1, off}
+    l0 = off["lowrank"]         # first low-rank block {r0, r1, c0, c1, k, uoff, voff, vx0, snode}
+    if field == "tmap_dup":
+        struct.pack_into("<q", raw, off["tmap"], struct.unpack_from("<q", raw, off["tmap"] + 8)[0])
+        match = "tmap"
+    elif field == "dense_rows":
+        struct.pack_into("<q", raw, d0 + 8, m + 1)
+        match = "dense block range"
+    elif field == "dense_off":
+        struct.pack_into("<q", raw, d0 + 32, struct.unpack_from("<q", raw, 40)[0])  # = dlen
+        match = "dense block offset"
+    elif field == "lr_rank":
+        struct.pack_into("<q", raw, l0 + 32, 1 << 40)
+        match = "low-rank block rank"
+    elif field == "lr_voff":
+        struct.pack_into("<q", raw, l0 + 48, counts[5])  # voff = len(V)
+        match = "low-rank block offsets"
+    elif field == "lr_vx0":
+        struct.pack_into("<q", raw, l0 + 56, struct.unpack_from("<q", raw, 32)[0])  # vx0 = R
+        match = "Vx rows"
+    else:
+        # a second block claiming the first block's source cluster with other columns
+        c1 = struct.unpack_from("<q", raw, l0 + 24)[0]
+        snode = struct.unpack_from("<q", raw, l0 + 64)[0]
+        for b in range(1, counts[3]):
+            e = l0 + 72 * b
+            if struct.unpack_from("<q", raw, e + 24)[0] != c1:
+                struct.pack_into("<q", raw, e + 64, snode)
+                break
+        else:
+            pytest.skip("all blocks share one column range")
+        match = "source cluster columns"
+    _expect_refused(_reseal(bytes(raw)), tmp_path, match)
+
+
+def test_save_arguments(laplace, tmp_path):
+    with pytest.raises(H.HMXError, match="unknown flags"):
+        from hmatrixgpu import _lib as L
+        L.check(L.lib.hmx_hmatrix_save(laplace._h, os.fsencode(tmp_path / "x.hmx"), 4))
+    with pytest.raises(H.HMXError, match="cannot open"):
+        H.HMatrix.load(tmp_path / "missing.hmx", storage=())
