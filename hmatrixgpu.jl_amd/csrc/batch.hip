@@ -48,6 +48,7 @@ struct Batch {
     FarItem* dA = nullptr;
     ReduceGroup* dG = nullptr;
     PairItem* dB = nullptr;
+    LeafGroup* dLG = nullptr;
     StageArrays arrays;
     InstDesc* dInst = nullptr;
     std::vector<InstDesc> hInst;  // host image of the table
@@ -66,11 +67,12 @@ struct Batch {
 namespace {
 
 void free_lists(Batch& b) {
-    for (void* p : {(void*)b.dA, (void*)b.dG, (void*)b.dB})
+    for (void* p : {(void*)b.dA, (void*)b.dG, (void*)b.dB, (void*)b.dLG})
         if (p) (void)hipFree(p);
     b.dA = nullptr;
     b.dG = nullptr;
     b.dB = nullptr;
+    b.dLG = nullptr;
 }
 
 void drop_graph(Batch& b) {
@@ -92,6 +94,7 @@ ScratchW& batch_scratch(Batch& b, size_t i, int W) {
         HIPCHECK(hipMalloc(&s.S, sb));
         HIPCHECK(hipMemset(s.S, 0, sb));
         HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
+        HIPCHECK(hipMalloc(&s.LP, std::max<int64_t>(op.nlpartial, 1) * W * p_elem(op.storage)));
     }
     return s;
 }
@@ -103,6 +106,7 @@ void build_lists(Batch& b) {
     std::vector<FarItem> A;
     std::vector<ReduceGroup> G;
     std::vector<PairItem> B;
+    std::vector<LeafGroup> LG;
     int64_t gmax = 0;
     b.hInst.assign(b.ops.size(), InstDesc{});
     for (size_t i = 0; i < b.ops.size(); i++) {
@@ -119,6 +123,10 @@ void build_lists(Batch& b) {
             it.inst = static_cast<int32_t>(i);
             B.push_back(it);
         }
+        for (LeafGroup g : op.hLGroups) {
+            g.inst = static_cast<int32_t>(i);
+            LG.push_back(g);
+        }
         gmax = std::max(gmax, op.n);
         ScratchW& sc = batch_scratch(b, i, b.cfg.W);
         InstDesc& d = b.hInst[i];
@@ -126,6 +134,7 @@ void build_lists(Batch& b) {
         d.colidx = op.colidx;
         d.S = sc.S;
         d.partial = sc.P;
+        d.lpartial = sc.LP;
         d.tmap0 = op.tmap0;
         d.inv_smap = op.inv_smap;
         d.n = op.n;
@@ -141,6 +150,7 @@ void build_lists(Batch& b) {
     b.dA = upload(A);
     b.dG = upload(G);
     b.dB = upload(B);
+    b.dLG = upload(LG);
     b.arrays = StageArrays{};
     b.arrays.itemsA = b.dA;
     b.arrays.nA = static_cast<int64_t>(A.size());
@@ -148,6 +158,8 @@ void build_lists(Batch& b) {
     b.arrays.nG = static_cast<int64_t>(G.size());
     b.arrays.itemsB = b.dB;
     b.arrays.nB = static_cast<int64_t>(B.size());
+    b.arrays.lgroups = b.dLG;
+    b.arrays.nLG = static_cast<int64_t>(LG.size());
     b.arrays.gather_n = gmax;
     b.arrays.gather_insts = static_cast<int>(b.ops.size());
     b.table_dirty = true;
@@ -273,7 +285,7 @@ Batch::~Batch() {
         if (p) (void)hipHostFree(p);
     for (auto& v : scratch)
         for (auto& s : v)
-            for (void* p : {s.S, s.P})
+            for (void* p : {s.S, s.P, s.LP})
                 if (p) (void)hipFree(p);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
 }

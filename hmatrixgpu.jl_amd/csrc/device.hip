@@ -64,6 +64,60 @@ int64_t leaf_item_work(const PairItem& it) {
     return a * 64 + 4096;
 }
 
+// Cut leaf items whose panels exceed max(total / kLeafPieces, kMinPiece) bytes
+// into pieces along k (each of the shared and private k-tile ranges split in
+// proportion), so that the launch has several items per resident workgroup
+// and no single item outlasts the rest.  Pieces write partial rows; the
+// returned groups sum them in piece order.  Returns the partial rows used.
+constexpr int64_t kLeafPieces = 16384, kMinPiece = int64_t(1) << 20;
+constexpr int kMaxPieces = 64;
+
+int64_t split_leaf_items(std::vector<PairItem>& items, int TK, int64_t pieces,
+                         std::vector<LeafGroup>& groups) {
+    int64_t total = 0;
+    for (const PairItem& it : items) total += leaf_item_work(it);
+    const int64_t piece = std::max(total / pieces, kMinPiece);
+    std::vector<PairItem> out;
+    out.reserve(items.size());
+    int64_t nrows = 0;
+    for (const PairItem& it : items) {
+        const int64_t c = std::min<int64_t>(kMaxPieces, (leaf_item_work(it) + piece - 1) / piece);
+        if (c <= 1) {
+            out.push_back(it);
+            continue;
+        }
+        const int32_t rows = it.rows[0] + it.rows[1];
+        LeafGroup g{};
+        g.pbase = nrows;
+        g.out_off[0] = it.out_off[0];
+        g.out_off[1] = it.out_off[1];
+        g.rows[0] = it.rows[0];
+        g.rows[1] = it.rows[1];
+        g.nparts = static_cast<int32_t>(c);
+        g.inst = it.inst;
+        groups.push_back(g);
+        auto cut = [c](int32_t kt, int64_t p) { return static_cast<int32_t>(kt * p / c); };
+        for (int64_t p = 0; p < c; p++) {
+            PairItem q = it;
+            const int32_t s0 = cut(it.sh_kt, p), s1 = cut(it.sh_kt, p + 1);
+            q.sh_kt = s1 - s0;
+            q.sh_col = it.sh_col + int64_t(s0) * TK;
+            for (int gi = 0; gi < 2; gi++) {
+                const int32_t a0 = cut(it.pr_kt[gi], p), a1 = cut(it.pr_kt[gi], p + 1);
+                q.pr_kt[gi] = a1 - a0;
+                q.pr_col[gi] = it.pr_col[gi] + int64_t(a0) * TK;
+                q.sh_data[gi] = it.sh_data[gi] + int64_t(s0) * TK * it.rows[gi];
+                q.pr_data[gi] = it.pr_data[gi] + int64_t(a0) * TK * it.rows[gi];
+            }
+            q.part = nrows;
+            nrows += rows;
+            out.push_back(q);
+        }
+    }
+    items.swap(out);
+    return nrows;
+}
+
 void order_leaf_items(std::vector<PairItem>& items, const char* env) {
     if (items.size() < 2 * kXcds) return;
     auto work = leaf_item_work;
@@ -114,11 +168,11 @@ void device_destroy(DeviceOp* op) {
     if (!op) return;
     (void)hipSetDevice(op->device);
     for (void* p : {op->panel, (void*)op->itemsB, (void*)op->colidx, (void*)op->itemsA,
-                    (void*)op->groups, (void*)op->tmap0, (void*)op->inv_smap})
+                    (void*)op->groups, (void*)op->lgroups, (void*)op->tmap0, (void*)op->inv_smap})
         if (p) (void)hipFree(p);
     for (auto& kv : op->scratch)
         for (auto& w : kv.second.w)
-            for (void* p : {w.S, w.P})
+            for (void* p : {w.S, w.P, w.LP})
                 if (p) (void)hipFree(p);
     for (auto& e : op->ev)
         for (auto& pr : e) {
@@ -135,7 +189,8 @@ int64_t device_index_bytes(const DeviceOp* op) {
     if (!op) return 0;
     return op->ncol * 4 + op->nA * static_cast<int64_t>(sizeof(FarItem)) +
            op->nB * static_cast<int64_t>(sizeof(PairItem)) +
-           op->nG * static_cast<int64_t>(sizeof(ReduceGroup)) + (op->m + op->n) * 4;
+           op->nG * static_cast<int64_t>(sizeof(ReduceGroup)) +
+           op->nLG * static_cast<int64_t>(sizeof(LeafGroup)) + (op->m + op->n) * 4;
 }
 
 // Build the device operator for one storage precision.
@@ -361,6 +416,7 @@ void device_build(HMatrix& h, int storage) {
     };
     for (const auto& gp : groups_b) {
         PairItem it{};
+        it.part = -1;
         const Unit& A = units[gp.first];
         it.out_off[0] = A.r0;
         it.rows[0] = A.rows;
@@ -464,9 +520,15 @@ void device_build(HMatrix& h, int storage) {
                     (void*)dgen, (void*)dvrow, (void*)dA, (void*)dB})
         if (p) (void)hipFree(p);
 
+    // HMX_LEAF_PIECES: target piece count of the split (0 keeps every leaf item whole)
+    std::vector<LeafGroup> lgroups;
+    const char* pieces_env = std::getenv("HMX_LEAF_PIECES");
+    const int64_t pieces = pieces_env ? std::atoll(pieces_env) : kLeafPieces;
+    if (pieces > 0) op->nlpartial = split_leaf_items(itemsB, TK, pieces, lgroups);
     op->hItemsA = itemsF;  // tree order, for batched operators
     op->hItemsB = itemsB;
     op->hGroups = groups;
+    op->hLGroups = lgroups;
     order_far_items(itemsF);
     order_leaf_items(itemsB, std::getenv("HMX_ITEM_ORDER"));
     op->itemsA = upload(itemsF);
@@ -475,6 +537,8 @@ void device_build(HMatrix& h, int storage) {
     op->nB = static_cast<int64_t>(itemsB.size());
     op->groups = upload(groups);
     op->nG = static_cast<int64_t>(groups.size());
+    op->lgroups = upload(lgroups);
+    op->nLG = static_cast<int64_t>(lgroups.size());
     h.dev[storage].reset(op.release());
 }
 
@@ -489,6 +553,7 @@ ScratchW& get_scratch(DeviceOp& op, hipStream_t stream, int W) {
         HIPCHECK(hipMalloc(&s.S, sb));
         HIPCHECK(hipMemset(s.S, 0, sb));  // zero row and padding rows
         HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
+        HIPCHECK(hipMalloc(&s.LP, std::max<int64_t>(op.nlpartial, 1) * W * p_elem(op.storage)));
         if (std::getenv("HMX_VERBOSE"))
             std::fprintf(stderr, "[hmx] storage %d W %d source/T rows %p (%zu bytes), partials %p\n",
                          op.storage, W, s.S, sb, s.P);
@@ -511,6 +576,7 @@ void run(DeviceOp& op, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrh
     one.colidx = op.colidx;
     one.S = sc.S;
     one.partial = sc.P;
+    one.lpartial = sc.LP;
     one.tmap0 = op.tmap0;
     one.inv_smap = op.inv_smap;
     one.X = X;
@@ -527,6 +593,8 @@ void run(DeviceOp& op, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrh
     A.nG = op.nG;
     A.itemsB = op.itemsB;
     A.nB = op.nB;
+    A.lgroups = op.lgroups;
+    A.nLG = op.nLG;
     A.gather_n = op.n;
     A.gather_insts = 1;
     for (int rhs0 = 0; rhs0 < nrhs; rhs0 += W) {

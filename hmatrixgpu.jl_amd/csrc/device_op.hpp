@@ -8,7 +8,8 @@ namespace hmx {
 
 struct ScratchW {
     void* S = nullptr;  // (n + R + 1 + TK) x W source rows
-    void* P = nullptr;  // far-field partial rows x W
+    void* P = nullptr;   // far-field partial rows x W
+    void* LP = nullptr;  // leaf-piece partial rows x W
 };
 
 struct Scratch {
@@ -31,12 +32,16 @@ struct DeviceOp {
     ReduceGroup* groups = nullptr;
     int64_t nG = 0;
     int64_t npartial = 0;
+    LeafGroup* lgroups = nullptr;
+    int64_t nLG = 0;
+    int64_t nlpartial = 0;
     int32_t* tmap0 = nullptr;
     int32_t* inv_smap = nullptr;
     // host copies of the work lists (the batched operator re-packs them)
     std::vector<PairItem> hItemsB;
     std::vector<FarItem> hItemsA;
     std::vector<ReduceGroup> hGroups;
+    std::vector<LeafGroup> hLGroups;
     std::mutex mu;
     std::map<void*, Scratch> scratch;
     bool profiling = false;
@@ -80,6 +85,8 @@ struct StageArrays {
     int64_t nG = 0;
     const PairItem* itemsB = nullptr;
     int64_t nB = 0;
+    const LeafGroup* lgroups = nullptr;
+    int64_t nLG = 0;
     int64_t gather_n = 0;   // largest source size over the instances
     int gather_insts = 1;   // instances (grid.y of the gather)
 };
@@ -104,8 +111,10 @@ struct StageTimer {
     }
 };
 
-// The four stages of a product pass: gather X -> S, far-field panels (V*X),
-// partial reduce, leaf panels -> Y.
+// The stages of a product pass: gather X -> S, far-field panels (V*X),
+// partial reduce, leaf panels -> Y (or leaf-piece partials), and the sum of
+// the leaf pieces -> Y.  The piece sum is timed with the far-field partial
+// reduce (stage 2), so stage 3 is the leaf kernel alone.
 template <typename TS, int W, typename TIO, bool BATCH>
 void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* insts,
                    hipStream_t st, DeviceOp* prof) {
@@ -129,6 +138,11 @@ void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* in
         StageTimer t(prof, st, 3);
         hipLaunchKernelGGL((k_leaf<TS, W, kPanelPolicy, TIO, BATCH>), dim3(A.nB),
                            dim3(kWaves * 64), 0, st, A.itemsB, one, insts);
+    }
+    if (A.nLG) {
+        StageTimer t(prof, st, 2);
+        hipLaunchKernelGGL((k_leaf_reduce<TS, W, TIO, BATCH>), dim3(A.nLG), dim3(256), 0, st,
+                           A.lgroups, one, insts);
     }
 }
 

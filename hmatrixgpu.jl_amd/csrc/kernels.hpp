@@ -60,6 +60,17 @@ struct PairItem {
     int32_t sh_kt;       // k-tiles of the shared segment
     int32_t pr_kt[2];    // k-tiles of each private part
     int32_t inst;        // instance (H-matrix) index in a batched launch
+    int64_t part;        // -1: write Y; else first leaf-partial row of this piece
+};
+
+// A leaf item cut along k into nparts pieces: the pieces' partial rows
+// (rows[0] + rows[1] each, back to back from pbase) are summed in piece order.
+struct LeafGroup {
+    int64_t pbase;
+    int64_t out_off[2];
+    int32_t rows[2];
+    int32_t nparts;
+    int32_t inst;
 };
 
 struct ReduceGroup {
@@ -79,6 +90,7 @@ struct InstDesc {
     const int32_t* colidx;
     void* S;                 // source rows: X permuted, then T, then a zero row
     void* partial;           // far-field column-chunk partial sums
+    void* lpartial;          // leaf-piece partial sums
     const int32_t* tmap0;    // permuted target row -> 0-based original row
     const int32_t* inv_smap; // 0-based original source row -> permuted row
     const void* X;
@@ -371,7 +383,12 @@ __device__ __forceinline__ void panel_accum(const TS* pdata, int rows, int ktile
     // register copies: while stage A is consumed, stage B (next k-tile) and the
     // column indices of the k-tile after it are in flight.  Loads past the end
     // of the panel (kt >= ktiles) fall outside the buffer and return zeros;
-    // they are never consumed.
+    // they are never consumed.  The loop runs over whole pairs of this wave's
+    // k-tiles and has a single exit, with an odd last k-tile after it: the
+    // accumulators then stay in place across the exit.  (A break between the
+    // two halves made the compiler copy just-written FP64 MFMA results at the
+    // exit with too few wait states, corrupting their low bits.)
+    const int mine = kt0 < ktiles ? (ktiles - kt0 + kstep - 1) / kstep : 0;
     int kt = kt0;
     IdxV<EPL> ixA{}, ixB{};
     if constexpr (LEAF) {
@@ -379,17 +396,16 @@ __device__ __forceinline__ void panel_accum(const TS* pdata, int rows, int ktile
         ixB = load_idx<EPL, TK>(Cr, kt + kstep, lg);
     }
     Stage sA = load_stage(kt, ixA), sB;
-    while (kt < ktiles) {
+    for (int i = 0; i + 1 < mine; i += 2) {
         if constexpr (LEAF) ixA = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
         sB = load_stage(kt + kstep, ixB);
         compute(sA);
-        kt += kstep;
-        if (kt >= ktiles) break;
-        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 2 * kstep, lg);
-        sA = load_stage(kt + kstep, ixA);
+        if constexpr (LEAF) ixB = load_idx<EPL, TK>(Cr, kt + 3 * kstep, lg);
+        sA = load_stage(kt + 2 * kstep, ixA);
         compute(sB);
-        kt += kstep;
+        kt += 2 * kstep;
     }
+    if (mine & 1) compute(sA);
 }
 
 // Hand the accumulated tile results to emit(row, col, value), row < 16*NRT.
@@ -468,11 +484,20 @@ __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict
     using s_t = typename SType<TS, W>::type;
     const PairItem it = items[blockIdx.x];
     const InstDesc& I = inst_of<BATCH>(one, insts, it.inst);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The wave index is uniform across the wave; readfirstlane tells the
+    // compiler so, which keeps every per-wave panel pointer and buffer
+    // descriptor in SGPRs (a VGPR descriptor wraps each load in a loop).
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const bool pair = it.rows[1] > 0;
     const int g = pair ? wave >> 1 : 0;
     const int kt0 = pair ? (wave & 1) : wave, kstep = pair ? 2 : 4;
-    const int rows = it.rows[g];
+    // per-row-group fields by select, not by a runtime index into the item
+    const int rows = g ? it.rows[1] : it.rows[0];
+    const int64_t sh_data = g ? it.sh_data[1] : it.sh_data[0];
+    const int64_t pr_data = g ? it.pr_data[1] : it.pr_data[0];
+    const int64_t pr_col = g ? it.pr_col[1] : it.pr_col[0];
+    const int pr_kt = g ? it.pr_kt[1] : it.pr_kt[0];
     __shared__ acc_t red[kWaves][64][W];
     auto emit = [&](int row, int j, acc_t v) { red[wave][row][j] = v; };
     const TS* panel = static_cast<const TS*>(I.panel);
@@ -482,14 +507,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict
         PanelAcc<TS, W> acc[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; rt++) acc[rt] = PanelAcc<TS, W>{};
-        panel_accum<TS, W, true, NRT, CP>(panel + it.sh_data[g], rows, it.sh_kt, it.sh_col, 0,
+        panel_accum<TS, W, true, NRT, CP>(panel + sh_data, rows, it.sh_kt, it.sh_col, 0,
                                           I.colidx, S, I.s_bytes, kt0, kstep, lane, acc);
-        panel_accum<TS, W, true, NRT, CP>(panel + it.pr_data[g], rows, it.pr_kt[g], it.pr_col[g], 0,
-                                          I.colidx, S, I.s_bytes, kt0, kstep, lane, acc);
+        panel_accum<TS, W, true, NRT, CP>(panel + pr_data, rows, pr_kt, pr_col, 0, I.colidx, S,
+                                          I.s_bytes, kt0, kstep, lane, acc);
         panel_emit<TS, W, NRT>(acc, lane, emit);
     });
     __syncthreads();
     TIO* Y = static_cast<TIO*>(I.Y);
+    acc_t* lpart = static_cast<acc_t*>(I.lpartial);
     const int r0 = it.rows[0], total = r0 + it.rows[1];
     for (int t = threadIdx.x; t < total * W; t += kWaves * 64) {
         const int rr = t / W, j = t % W;
@@ -502,9 +528,34 @@ __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict
 #pragma unroll
             for (int w = 1; w < kWaves; w++) s += red[w][r][j];
         }
-        if (I.rhs0 + j < I.nrhs)
-            Y[static_cast<int64_t>(I.tmap0[it.out_off[gg] + r]) +
+        if (it.part >= 0)
+            lpart[(it.part + rr) * W + j] = s;
+        else if (I.rhs0 + j < I.nrhs)
+            Y[static_cast<int64_t>(I.tmap0[(gg ? it.out_off[1] : it.out_off[0]) + r]) +
               static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(s);
+    }
+}
+
+// Sum the pieces of a split leaf item in piece order and scatter to Y.
+template <typename TS, int W, typename TIO, bool BATCH>
+__global__ __launch_bounds__(256) void k_leaf_reduce(const LeafGroup* __restrict__ groups,
+                                                     const InstDesc one,
+                                                     const InstDesc* __restrict__ insts) {
+    using acc_t = typename Store<TS>::acc;
+    const LeafGroup g = groups[blockIdx.x];
+    const InstDesc& I = inst_of<BATCH>(one, insts, g.inst);
+    const acc_t* lpart = static_cast<const acc_t*>(I.lpartial);
+    TIO* Y = static_cast<TIO*>(I.Y);
+    const int r0 = g.rows[0], total = r0 + g.rows[1];
+    for (int t = threadIdx.x; t < total * W; t += blockDim.x) {
+        const int rr = t / W, j = t % W;
+        if (I.rhs0 + j >= I.nrhs) continue;
+        acc_t s = lpart[(g.pbase + rr) * W + j];
+        for (int p = 1; p < g.nparts; p++)
+            s += lpart[(g.pbase + static_cast<int64_t>(p) * total + rr) * W + j];
+        const int gg = rr < r0 ? 0 : 1, r = rr - gg * r0;
+        Y[static_cast<int64_t>(I.tmap0[(gg ? g.out_off[1] : g.out_off[0]) + r]) +
+          static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(s);
     }
 }
 
@@ -535,7 +586,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_far(const FarItem* __restrict__
     constexpr int TK = Store<TS>::TK;
     const FarItem it = items[blockIdx.x];
     const InstDesc& I = inst_of<BATCH>(one, insts, it.inst);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // uniform
     if (wave >= it.nrg) return;
     const int rows = wave == it.nrg - 1 ? it.rows_last : 64;
     const TS* pd = static_cast<const TS*>(I.panel) + it.data_off +
