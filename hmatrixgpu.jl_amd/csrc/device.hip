@@ -325,6 +325,23 @@ void device_build(HMatrix& h, int storage) {
         std::set_difference(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
         return out;
     };
+    // Dense blocks of two row groups that cover the same source columns (the
+    // same block, or e.g. the blocks of sibling leaves with one neighbour
+    // leaf): their columns can share one read of S.  The block tree gives each
+    // (row, column) one block, so a source range appears at most once per row
+    // group.  Returns the matched blocks of a and of b (in a's order).
+    auto match_dense = [&](const std::vector<int32_t>& a, const std::vector<int32_t>& b) {
+        std::map<std::pair<int64_t, int64_t>, int32_t> by_cols;
+        for (int32_t d : b) by_cols[{h.dense[d].c0, h.dense[d].c1}] = d;
+        std::pair<std::vector<int32_t>, std::vector<int32_t>> m;
+        for (int32_t d : a) {
+            auto f = by_cols.find({h.dense[d].c0, h.dense[d].c1});
+            if (f == by_cols.end()) continue;
+            m.first.push_back(d);
+            m.second.push_back(f->second);
+        }
+        return m;
+    };
     auto ncols_of = [&](const std::vector<int32_t>& dl, const std::vector<int32_t>& ll) {
         int64_t c = 0;
         for (int32_t d : dl) c += h.dense[d].c1 - h.dense[d].c0 + 1;
@@ -337,7 +354,7 @@ void device_build(HMatrix& h, int storage) {
     std::vector<int64_t> gain(static_cast<size_t>(std::max<int64_t>(nu, 1)), 0);
     for (int64_t u = 0; pairing && u + 1 < nu; u++) {
         const int64_t la = units[u].leaf, lb = units[u + 1].leaf;
-        const int64_t c = ncols_of(intersect(dense_of[la], dense_of[lb]),
+        const int64_t c = ncols_of(match_dense(dense_of[la], dense_of[lb]).first,
                                    intersect(lr_of[la], lr_of[lb]));
         gain[u] = c >= 2 * TK ? c : 0;  // too few shared columns: not worth a segment
     }
@@ -429,16 +446,18 @@ void device_build(HMatrix& h, int storage) {
             const Unit& B = units[gp.second];
             it.out_off[1] = B.r0;
             it.rows[1] = B.rows;
-            const auto sd = intersect(dense_of[A.leaf], dense_of[B.leaf]);
+            auto sd = match_dense(dense_of[A.leaf], dense_of[B.leaf]);
             const auto sl = intersect(lr_of[A.leaf], lr_of[B.leaf]);
-            auto [sc, skt] = add_colidx(sd, sl);
+            auto [sc, skt] = add_colidx(sd.first, sl);
             it.sh_col = sc;
             it.sh_kt = skt;
-            it.sh_data[0] = add_panel(sd, sl, A);
-            it.sh_data[1] = add_panel(sd, sl, B);
+            it.sh_data[0] = add_panel(sd.first, sl, A);
+            it.sh_data[1] = add_panel(sd.second, sl, B);
             const Unit* UU[2] = {&A, &B};
+            std::vector<int32_t>* sdq[2] = {&sd.first, &sd.second};
             for (int q = 0; q < 2; q++) {
-                const auto pd = minus(dense_of[UU[q]->leaf], sd);
+                std::sort(sdq[q]->begin(), sdq[q]->end());
+                const auto pd = minus(dense_of[UU[q]->leaf], *sdq[q]);
                 const auto pl = minus(lr_of[UU[q]->leaf], sl);
                 auto [pc, pkt] = add_colidx(pd, pl);
                 it.pr_col[q] = pc;
