@@ -95,6 +95,7 @@ ScratchW& batch_scratch(Batch& b, size_t i, int W) {
         HIPCHECK(hipMemset(s.S, 0, sb));
         HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
         HIPCHECK(hipMalloc(&s.LP, std::max<int64_t>(op.nlpartial, 1) * W * p_elem(op.storage)));
+        HIPCHECK(hipMalloc(&s.Yp, std::max<int64_t>(op.m, 1) * W * p_elem(op.storage)));
     }
     return s;
 }
@@ -107,7 +108,7 @@ void build_lists(Batch& b) {
     std::vector<ReduceGroup> G;
     std::vector<PairItem> B;
     std::vector<LeafGroup> LG;
-    int64_t gmax = 0;
+    int64_t gmax = 0, mmax = 0;
     b.hInst.assign(b.ops.size(), InstDesc{});
     for (size_t i = 0; i < b.ops.size(); i++) {
         const DeviceOp& op = *b.ops[i];
@@ -128,6 +129,7 @@ void build_lists(Batch& b) {
             LG.push_back(g);
         }
         gmax = std::max(gmax, op.n);
+        mmax = std::max(mmax, op.m);
         ScratchW& sc = batch_scratch(b, i, b.cfg.W);
         InstDesc& d = b.hInst[i];
         d.panel = op.panel;
@@ -135,6 +137,9 @@ void build_lists(Batch& b) {
         d.S = sc.S;
         d.partial = sc.P;
         d.lpartial = sc.LP;
+        d.Yp = sc.Yp;
+        d.inv_tmap = op.inv_tmap;
+        d.m = op.m;
         d.tmap0 = op.tmap0;
         d.inv_smap = op.inv_smap;
         d.n = op.n;
@@ -162,6 +167,7 @@ void build_lists(Batch& b) {
     b.arrays.nLG = static_cast<int64_t>(LG.size());
     b.arrays.gather_n = gmax;
     b.arrays.gather_insts = static_cast<int>(b.ops.size());
+    b.arrays.out_m = mmax;
     b.table_dirty = true;
 }
 
@@ -285,7 +291,7 @@ Batch::~Batch() {
         if (p) (void)hipHostFree(p);
     for (auto& v : scratch)
         for (auto& s : v)
-            for (void* p : {s.S, s.P, s.LP})
+            for (void* p : {s.S, s.P, s.LP, s.Yp})
                 if (p) (void)hipFree(p);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
 }

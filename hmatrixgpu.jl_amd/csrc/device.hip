@@ -168,11 +168,12 @@ void device_destroy(DeviceOp* op) {
     if (!op) return;
     (void)hipSetDevice(op->device);
     for (void* p : {op->panel, (void*)op->itemsB, (void*)op->colidx, (void*)op->itemsA,
-                    (void*)op->groups, (void*)op->lgroups, (void*)op->tmap0, (void*)op->inv_smap})
+                    (void*)op->groups, (void*)op->lgroups, (void*)op->tmap0, (void*)op->inv_tmap,
+                    (void*)op->inv_smap})
         if (p) (void)hipFree(p);
     for (auto& kv : op->scratch)
         for (auto& w : kv.second.w)
-            for (void* p : {w.S, w.P, w.LP})
+            for (void* p : {w.S, w.P, w.LP, w.Yp})
                 if (p) (void)hipFree(p);
     for (auto& e : op->ev)
         for (auto& pr : e) {
@@ -190,7 +191,7 @@ int64_t device_index_bytes(const DeviceOp* op) {
     return op->ncol * 4 + op->nA * static_cast<int64_t>(sizeof(FarItem)) +
            op->nB * static_cast<int64_t>(sizeof(PairItem)) +
            op->nG * static_cast<int64_t>(sizeof(ReduceGroup)) +
-           op->nLG * static_cast<int64_t>(sizeof(LeafGroup)) + (op->m + op->n) * 4;
+           op->nLG * static_cast<int64_t>(sizeof(LeafGroup)) + (2 * op->m + op->n) * 4;
 }
 
 // Build the device operator for one storage precision.
@@ -471,10 +472,12 @@ void device_build(HMatrix& h, int storage) {
     op->npartial = npartial;
 
     // -- upload and generate ---------------------------------------------------------
-    std::vector<int32_t> tmap0(h.m), inv(h.n);
+    std::vector<int32_t> tmap0(h.m), inv_t(h.m), inv(h.n);
     for (int64_t i = 0; i < h.m; i++) tmap0[i] = static_cast<int32_t>(h.tmap[i] - 1);
+    for (int64_t i = 0; i < h.m; i++) inv_t[h.tmap[i] - 1] = static_cast<int32_t>(i);
     for (int64_t i = 0; i < h.n; i++) inv[h.smap[i] - 1] = static_cast<int32_t>(i);
     op->tmap0 = upload(tmap0);
+    op->inv_tmap = upload(inv_t);
     op->inv_smap = upload(inv);
     // The FP64 staging arrays are allocated before the panel and freed after
     // generation.  Measured on the MI355X, the product's speed depends on where
@@ -573,6 +576,7 @@ ScratchW& get_scratch(DeviceOp& op, hipStream_t stream, int W) {
         HIPCHECK(hipMemset(s.S, 0, sb));  // zero row and padding rows
         HIPCHECK(hipMalloc(&s.P, std::max<int64_t>(op.npartial, 1) * W * p_elem(op.storage)));
         HIPCHECK(hipMalloc(&s.LP, std::max<int64_t>(op.nlpartial, 1) * W * p_elem(op.storage)));
+        HIPCHECK(hipMalloc(&s.Yp, std::max<int64_t>(op.m, 1) * W * p_elem(op.storage)));
         if (std::getenv("HMX_VERBOSE"))
             std::fprintf(stderr, "[hmx] storage %d W %d source/T rows %p (%zu bytes), partials %p\n",
                          op.storage, W, s.S, sb, s.P);
@@ -596,6 +600,9 @@ void run(DeviceOp& op, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrh
     one.S = sc.S;
     one.partial = sc.P;
     one.lpartial = sc.LP;
+    one.Yp = sc.Yp;
+    one.inv_tmap = op.inv_tmap;
+    one.m = op.m;
     one.tmap0 = op.tmap0;
     one.inv_smap = op.inv_smap;
     one.X = X;
@@ -616,6 +623,7 @@ void run(DeviceOp& op, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrh
     A.nLG = op.nLG;
     A.gather_n = op.n;
     A.gather_insts = 1;
+    A.out_m = op.m;
     for (int rhs0 = 0; rhs0 < nrhs; rhs0 += W) {
         one.rhs0 = rhs0;
         launch_stages<TS, W, TIO, false>(A, one, nullptr, st, &op);

@@ -10,6 +10,7 @@ struct ScratchW {
     void* S = nullptr;  // (n + R + 1 + TK) x W source rows
     void* P = nullptr;   // far-field partial rows x W
     void* LP = nullptr;  // leaf-piece partial rows x W
+    void* Yp = nullptr;  // leaf results in permuted target order, m x W
 };
 
 struct Scratch {
@@ -36,6 +37,7 @@ struct DeviceOp {
     int64_t nLG = 0;
     int64_t nlpartial = 0;
     int32_t* tmap0 = nullptr;
+    int32_t* inv_tmap = nullptr;
     int32_t* inv_smap = nullptr;
     // host copies of the work lists (the batched operator re-packs them)
     std::vector<PairItem> hItemsB;
@@ -88,7 +90,8 @@ struct StageArrays {
     const LeafGroup* lgroups = nullptr;
     int64_t nLG = 0;
     int64_t gather_n = 0;   // largest source size over the instances
-    int gather_insts = 1;   // instances (grid.y of the gather)
+    int gather_insts = 1;   // instances (grid.y of the gather and the unpermute)
+    int64_t out_m = 0;      // largest target size over the instances
 };
 
 // HIP-event timing of one stage on the launch stream (only when profiling).
@@ -112,9 +115,10 @@ struct StageTimer {
 };
 
 // The stages of a product pass: gather X -> S, far-field panels (V*X),
-// partial reduce, leaf panels -> Y (or leaf-piece partials), and the sum of
-// the leaf pieces -> Y.  The piece sum is timed with the far-field partial
-// reduce (stage 2), so stage 3 is the leaf kernel alone.
+// partial reduce, leaf panels -> permuted rows (or leaf-piece partials), the
+// sum of the leaf pieces, and the unpermute into Y.  The piece sum and the
+// unpermute are timed with the far-field partial reduce (stage 2), so stage 3
+// is the leaf kernel alone.
 template <typename TS, int W, typename TIO, bool BATCH>
 void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* insts,
                    hipStream_t st, DeviceOp* prof) {
@@ -136,14 +140,16 @@ void launch_stages(const StageArrays& A, const InstDesc& one, const InstDesc* in
     }
     if (A.nB) {
         StageTimer t(prof, st, 3);
-        hipLaunchKernelGGL((k_leaf<TS, W, kPanelPolicy, TIO, BATCH>), dim3(A.nB),
+        hipLaunchKernelGGL((k_leaf<TS, W, kPanelPolicy, BATCH>), dim3(A.nB),
                            dim3(kWaves * 64), 0, st, A.itemsB, one, insts);
     }
-    if (A.nLG) {
-        StageTimer t(prof, st, 2);
-        hipLaunchKernelGGL((k_leaf_reduce<TS, W, TIO, BATCH>), dim3(A.nLG), dim3(256), 0, st,
+    StageTimer t(prof, st, 2);
+    if (A.nLG)
+        hipLaunchKernelGGL((k_leaf_reduce<TS, W, BATCH>), dim3(A.nLG), dim3(256), 0, st,
                            A.lgroups, one, insts);
-    }
+    hipLaunchKernelGGL((k_unpermute<TS, W, TIO, BATCH>),
+                       dim3(static_cast<unsigned>((A.out_m + 255) / 256), A.gather_insts),
+                       dim3(256), 0, st, one, insts);
 }
 
 // Dispatch order of leaf work items (see device.hip).

@@ -91,6 +91,9 @@ struct InstDesc {
     void* S;                 // source rows: X permuted, then T, then a zero row
     void* partial;           // far-field column-chunk partial sums
     void* lpartial;          // leaf-piece partial sums
+    void* Yp;                // leaf results in permuted target order (m x W)
+    const int32_t* inv_tmap; // 0-based original target row -> permuted row
+    int64_t m;               // target rows
     const int32_t* tmap0;    // permuted target row -> 0-based original row
     const int32_t* inv_smap; // 0-based original source row -> permuted row
     const void* X;
@@ -238,6 +241,13 @@ template <typename T, int W>
 struct SRow {
     T v[W];
 };
+// One S (or output) row as a unit of store: W elements, aligned to its size (at most 16
+// bytes), so that it is written with row-sized vector stores.
+template <typename T, int W>
+struct alignas(sizeof(T) * W >= 16 ? 16 : sizeof(T) * W) SRowOut {
+    T v[W];
+};
+
 template <typename T, int W>
 __device__ inline SRow<T, W> load_srow(__amdgpu_buffer_rsrc_t R, uint32_t off) {
     SRow<T, W> r;
@@ -476,7 +486,7 @@ __device__ __forceinline__ const InstDesc& inst_of(const InstDesc& one, const In
 // row is fetched once per workgroup and served to the second wave from L1.  A
 // single row group is split over all four waves.  Partial sums are reduced
 // through LDS and scattered to Y through target_map.
-template <typename TS, int W, int CP, typename TIO, bool BATCH>
+template <typename TS, int W, int CP, bool BATCH>
 __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict__ items,
                                                       const InstDesc one,
                                                       const InstDesc* __restrict__ insts) {
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict
         panel_emit<TS, W, NRT>(acc, lane, emit);
     });
     __syncthreads();
-    TIO* Y = static_cast<TIO*>(I.Y);
+    acc_t* Yp = static_cast<acc_t*>(I.Yp);
     acc_t* lpart = static_cast<acc_t*>(I.lpartial);
     const int r0 = it.rows[0], total = r0 + it.rows[1];
     for (int t = threadIdx.x; t < total * W; t += kWaves * 64) {
@@ -530,14 +540,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict
         }
         if (it.part >= 0)
             lpart[(it.part + rr) * W + j] = s;
-        else if (I.rhs0 + j < I.nrhs)
-            Y[static_cast<int64_t>(I.tmap0[(gg ? it.out_off[1] : it.out_off[0]) + r]) +
-              static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(s);
+        else
+            Yp[((gg ? it.out_off[1] : it.out_off[0]) + r) * W + j] = s;
     }
 }
 
-// Sum the pieces of a split leaf item in piece order and scatter to Y.
-template <typename TS, int W, typename TIO, bool BATCH>
+// Sum the pieces of a split leaf item in piece order into the permuted rows.
+template <typename TS, int W, bool BATCH>
 __global__ __launch_bounds__(256) void k_leaf_reduce(const LeafGroup* __restrict__ groups,
                                                      const InstDesc one,
                                                      const InstDesc* __restrict__ insts) {
@@ -545,18 +554,33 @@ __global__ __launch_bounds__(256) void k_leaf_reduce(const LeafGroup* __restrict
     const LeafGroup g = groups[blockIdx.x];
     const InstDesc& I = inst_of<BATCH>(one, insts, g.inst);
     const acc_t* lpart = static_cast<const acc_t*>(I.lpartial);
-    TIO* Y = static_cast<TIO*>(I.Y);
+    acc_t* Yp = static_cast<acc_t*>(I.Yp);
     const int r0 = g.rows[0], total = r0 + g.rows[1];
     for (int t = threadIdx.x; t < total * W; t += blockDim.x) {
         const int rr = t / W, j = t % W;
-        if (I.rhs0 + j >= I.nrhs) continue;
         acc_t s = lpart[(g.pbase + rr) * W + j];
         for (int p = 1; p < g.nparts; p++)
             s += lpart[(g.pbase + static_cast<int64_t>(p) * total + rr) * W + j];
         const int gg = rr < r0 ? 0 : 1, r = rr - gg * r0;
-        Y[static_cast<int64_t>(I.tmap0[(gg ? g.out_off[1] : g.out_off[0]) + r]) +
-          static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(s);
+        Yp[((gg ? g.out_off[1] : g.out_off[0]) + r) * W + j] = s;
     }
+}
+
+// Y[i, rhs0 + j] = Yp[inv_tmap[i]][j] for the pass's columns; thread per
+// original target row (instance = blockIdx.y): one whole permuted row read,
+// writes coalesced along each column of Y.
+template <typename TS, int W, typename TIO, bool BATCH>
+__global__ __launch_bounds__(256) void k_unpermute(const InstDesc one,
+                                                   const InstDesc* __restrict__ insts) {
+    using acc_t = typename Store<TS>::acc;
+    const InstDesc& I = inst_of<BATCH>(one, insts, blockIdx.y);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= I.m) return;
+    const SRowOut<acc_t, W> row = static_cast<const SRowOut<acc_t, W>*>(I.Yp)[I.inv_tmap[i]];
+    TIO* Y = static_cast<TIO*>(I.Y);
+#pragma unroll
+    for (int j = 0; j < W; j++)
+        if (I.rhs0 + j < I.nrhs) Y[i + static_cast<int64_t>(I.rhs0 + j) * I.ldy] = static_cast<TIO>(row.v[j]);
 }
 
 // Far-field work item: up to kWaves consecutive 64-row groups of the stacked V
@@ -629,7 +653,8 @@ __global__ __launch_bounds__(256) void k_reduce(const ReduceGroup* __restrict__ 
 }
 
 // S[p][j] = X[smap[p], rhs0 + j]  (zero for j beyond nrhs); thread per original
-// source index, instance = blockIdx.y.
+// source index, instance = blockIdx.y.  Reads are coalesced per column of X;
+// each thread writes its S row whole.
 template <typename TS, int W, typename TIO, bool BATCH>
 __global__ __launch_bounds__(256) void k_gather(const InstDesc one, const InstDesc* __restrict__ insts) {
     using s_t = typename SType<TS, W>::type;
@@ -637,14 +662,15 @@ __global__ __launch_bounds__(256) void k_gather(const InstDesc one, const InstDe
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= I.n) return;
     const TIO* X = static_cast<const TIO*>(I.X);
-    s_t* S = static_cast<s_t*>(I.S);
     const int64_t p = I.inv_smap[t];
+    SRowOut<s_t, W> row;
 #pragma unroll
     for (int j = 0; j < W; j++) {
         const int c = I.rhs0 + j;
         const double v = c < I.nrhs ? static_cast<double>(X[t + static_cast<int64_t>(c) * I.ldx]) : 0.0;
-        S[p * W + j] = from_double<s_t>(v);
+        row.v[j] = from_double<s_t>(v);
     }
+    reinterpret_cast<SRowOut<s_t, W>*>(I.S)[p] = row;
 }
 
 // ---- build-time panel generation ------------------------------------------------
