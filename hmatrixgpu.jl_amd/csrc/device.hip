@@ -215,7 +215,14 @@ void device_build(HMatrix& h, int storage) {
     op->n = h.n;
     op->R = h.R;
     const int TK = tk_of(storage);
-    const int64_t CH = 2048;  // far-field column chunk (multiple of every TK)
+    // Far-field column chunk, a multiple of every TK.  Source clusters wider than
+    // a chunk write partial rows that k_reduce sums; 8192 halves those partials
+    // against 2048 (measured: the far kernel's time is the same, the reduce 2x
+    // shorter) while keeping items at most 4 x 64 x 8192 entries.  HMX_FAR_CHUNK
+    // overrides it.
+    const char* ch_env = std::getenv("HMX_FAR_CHUNK");
+    const int64_t ch_req = ch_env ? std::atoll(ch_env) : 8192;
+    const int64_t CH = ch_req >= 32 && ch_req % 32 == 0 ? ch_req : 8192;
 
     // -- far field: group low-rank blocks by source cluster -----------------------
     // Per source cluster the rank rows of its blocks are stacked (T rows, in
