@@ -248,6 +248,70 @@ __device__ double wave_dot(const double* x, int64_t sx, const double* y, int64_t
     return wave_fold(p);
 }
 
+// Apply the reflector (1, a[j+1..m)) with factor t to columns c0 .. c0+nc-1
+// (nc <= kReflCols) of B, element i of column c at B[c*cs + i*es]:
+//   w = (b[j] + sum_{i>j} a[i] b[i]) * t;  b[j] -= w;  b[i] -= a[i] w.
+// Every column keeps the operation order of one wave_dot and one update (lane
+// partial sums over i = lane + 64 s, then the shuffle fold), so the result is
+// the same bit for bit; the columns share one pass over a and one pair of
+// barriers instead of one each.
+constexpr int kReflCols = 8;
+
+template <int NC>
+__device__ __forceinline__ void reflect_cols_n(const double* a, int m, int j, double t, double* B,
+                                               int64_t cs, int64_t es, int c0, int lane) {
+#pragma clang fp contract(off)
+    double* col[NC];
+    double p[NC], bj[NC], w[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        col[q] = B + (c0 + q) * cs;
+        p[q] = 0.0;
+        bj[q] = col[q][j * es];
+    }
+    for (int i = j + 1 + lane; i < m; i += 64) {
+        const double ai = a[i];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const double pr = ai * col[q][i * es];
+            p[q] = p[q] + pr;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+        w[q] = bj[q] + wave_fold(p[q]);
+        w[q] = w[q] * t;
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) col[q][j * es] = bj[q] - w[q];
+    }
+    for (int i = j + 1 + lane; i < m; i += 64) {
+        const double ai = a[i];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            double* b = col[q] + i * es;
+            *b = *b - ai * w[q];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void reflect_cols(const double* a, int m, int j, double t, double* B,
+                                             int64_t cs, int64_t es, int c0, int nc, int lane) {
+    switch (nc) {
+    case 1: reflect_cols_n<1>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 2: reflect_cols_n<2>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 3: reflect_cols_n<3>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 4: reflect_cols_n<4>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 5: reflect_cols_n<5>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 6: reflect_cols_n<6>(a, m, j, t, B, cs, es, c0, lane); break;
+    case 7: reflect_cols_n<7>(a, m, j, t, B, cs, es, c0, lane); break;
+    default: reflect_cols_n<8>(a, m, j, t, B, cs, es, c0, lane); break;
+    }
+}
+
 __device__ void wave_qr(int m, int k, double* A, double* tau, int lane) {
 #pragma clang fp contract(off)
     for (int j = 0; j < k; j++) {
@@ -269,16 +333,8 @@ __device__ void wave_qr(int m, int k, double* A, double* tau, int lane) {
             a[j] = beta;
             tau[j] = t;
         }
-        for (int c = j + 1; c < k; c++) {
-            double* b = A + static_cast<int64_t>(c) * m;
-            const double bj = b[j];
-            double w = bj + wave_dot(a + j + 1, 1, b + j + 1, 1, m - j - 1, lane);
-            w = w * t;
-            __syncthreads();
-            if (lane == 0) b[j] = bj - w;
-            for (int i = j + 1 + lane; i < m; i += 64) b[i] = b[i] - a[i] * w;
-            __syncthreads();
-        }
+        for (int c = j + 1; c < k; c += kReflCols)
+            reflect_cols(a, m, j, t, A, m, 1, c, min(kReflCols, k - c), lane);
         __syncthreads();
     }
 }
@@ -291,16 +347,8 @@ __device__ void wave_apply_q(int m, int k, const double* A, const double* tau, i
         const double t = tau[j];
         if (t == 0.0) continue;
         const double* a = A + static_cast<int64_t>(j) * m;
-        for (int c = 0; c < ncols; c++) {
-            double* b = B + c * cs;
-            const double bj = b[j * es];
-            double w = bj + wave_dot(a + j + 1, 1, b + (j + 1) * es, es, m - j - 1, lane);
-            w = w * t;
-            __syncthreads();
-            if (lane == 0) b[j * es] = bj - w;
-            for (int i = j + 1 + lane; i < m; i += 64) b[i * es] = b[i * es] - a[i] * w;
-            __syncthreads();
-        }
+        for (int c = 0; c < ncols; c += kReflCols)
+            reflect_cols(a, m, j, t, B, cs, es, c, min(kReflCols, ncols - c), lane);
     }
 }
 
