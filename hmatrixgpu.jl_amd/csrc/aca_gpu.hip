@@ -352,28 +352,17 @@ __device__ void wave_apply_q(int m, int k, const double* A, const double* tau, i
     }
 }
 
-__global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, double* pool,
-                                                 double* __restrict__ scratch, double eps) {
+// R_U R_V' -> one-sided Jacobi SVD -> sorted singular values, W, Z2 and the
+// truncation rank.  M and Z (k x k) live in LDS when they fit (kLdsK) and in
+// the block's global scratch otherwise; the arithmetic and its order are the
+// same either way.
+constexpr int kLdsK = 32;
+
+template <bool IN_LDS>
+__device__ __forceinline__ void rec_svd(RecTask& T, const double* QU, const double* QV, double* M,
+                                        double* Z, double* W, double* Z2, double* nrm, double* sig,
+                                        double* ord, int m, int n, int k, double eps, int lane) {
 #pragma clang fp contract(off)
-    RecTask& T = tasks[blockIdx.x];
-    const int lane = threadIdx.x, m = T.m, n = T.n, k = T.k;
-    if (k == 0) {
-        if (lane == 0) T.r = 0;
-        return;
-    }
-    double* QU = pool + T.uoff;
-    double* QV = pool + T.voff;
-    double* M = scratch + T.soff;
-    double* Z = M + k * k;
-    double* W = Z + k * k;
-    double* Z2 = W + k * k;
-    double* nrm = Z2 + k * k;
-    double* sig = nrm + k;
-    double* ord = sig + k;
-    double* tu = ord + k;
-    double* tv = tu + k;
-    wave_qr(m, k, QU, tu, lane);
-    wave_qr(n, k, QV, tv, lane);
     for (int idx = lane; idx < k * k; idx += 64) {  // (R_U R_V')[i, j]
         const int i = idx % k, j = idx / k;
         double s = 0.0;
@@ -473,6 +462,35 @@ __global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, do
             }
         T.r = r;
     }
+}
+
+__global__ __launch_bounds__(64) void k_rec_core(RecTask* __restrict__ tasks, double* pool,
+                                                 double* __restrict__ scratch, double eps) {
+#pragma clang fp contract(off)
+    RecTask& T = tasks[blockIdx.x];
+    const int lane = threadIdx.x, m = T.m, n = T.n, k = T.k;
+    if (k == 0) {
+        if (lane == 0) T.r = 0;
+        return;
+    }
+    double* QU = pool + T.uoff;
+    double* QV = pool + T.voff;
+    double* M = scratch + T.soff;
+    double* Z = M + k * k;
+    double* W = Z + k * k;
+    double* Z2 = W + k * k;
+    double* nrm = Z2 + k * k;
+    double* sig = nrm + k;
+    double* ord = sig + k;
+    double* tu = ord + k;
+    double* tv = tu + k;
+    wave_qr(m, k, QU, tu, lane);
+    wave_qr(n, k, QV, tv, lane);
+    __shared__ double mz[2 * kLdsK * kLdsK];
+    if (k <= kLdsK)
+        rec_svd<true>(T, QU, QV, mz, mz + k * k, W, Z2, nrm, sig, ord, m, n, k, eps, lane);
+    else
+        rec_svd<false>(T, QU, QV, M, Z, W, Z2, nrm, sig, ord, m, n, k, eps, lane);
 }
 
 __global__ __launch_bounds__(64) void k_rec_apply(const RecTask* __restrict__ tasks,
