@@ -76,7 +76,11 @@ int64_t split_leaf_items(std::vector<PairItem>& items, int TK, int64_t pieces,
                          std::vector<LeafGroup>& groups) {
     int64_t total = 0;
     for (const PairItem& it : items) total += leaf_item_work(it);
-    const int64_t piece = std::max(total / pieces, kMinPiece);
+    // HMX_LEAF_MIN_PIECE lowers the minimum piece (bytes), e.g. to split the
+    // items of small operators in tests
+    const char* min_env = std::getenv("HMX_LEAF_MIN_PIECE");
+    const int64_t min_piece = min_env ? std::max<int64_t>(4096, std::atoll(min_env)) : kMinPiece;
+    const int64_t piece = std::max(total / pieces, min_piece);
     std::vector<PairItem> out;
     out.reserve(items.size());
     int64_t nrows = 0;
