@@ -124,30 +124,27 @@ int64_t split_leaf_items(std::vector<PairItem>& items, int TK, int64_t pieces,
     return nrows;
 }
 
-void order_leaf_items(std::vector<PairItem>& items, const char* env) {
-    if (items.size() < 2 * kXcds) return;
-    auto work = leaf_item_work;
-    if (!env || std::string(env) == "lpt") {
-        std::stable_sort(items.begin(), items.end(),
-                         [&](const PairItem& a, const PairItem& b) { return work(a) > work(b); });
-        return;
-    }
-    if (env && std::string(env) == "tree") return;
+// Cut the (tree-ordered) list into kXcds contiguous chunks of equal work and
+// interleave them, so that workgroup i, placed on XCD i % kXcds, takes items
+// of chunk i % kXcds (a chunk that runs out borrows from the fullest one).
+template <typename T, typename Work>
+void xcd_interleave(std::vector<T>& items, Work work) {
     int64_t total = 0;
-    for (const PairItem& it : items) total += work(it);
-    std::vector<std::vector<PairItem>> chunk(kXcds);
+    for (const T& it : items) total += work(it);
+    if (total <= 0) return;
+    std::vector<std::vector<T>> chunk(kXcds);
     int64_t acc = 0;
-    for (const PairItem& it : items) {
+    for (const T& it : items) {
         const int c = static_cast<int>(std::min<int64_t>(kXcds - 1, acc * kXcds / total));
         chunk[c].push_back(it);
         acc += work(it);
     }
     std::vector<size_t> pos(kXcds, 0);
-    std::vector<PairItem> out;
+    std::vector<T> out;
     out.reserve(items.size());
     for (size_t p = 0; out.size() < items.size(); p++) {
         int c = static_cast<int>(p % kXcds);
-        if (pos[c] == chunk[c].size()) {  // exhausted: borrow from the fullest chunk
+        if (pos[c] == chunk[c].size()) {
             size_t best = 0;
             for (int q = 0; q < kXcds; q++)
                 if (chunk[q].size() - pos[q] > chunk[best].size() - pos[best]) best = q;
@@ -158,10 +155,23 @@ void order_leaf_items(std::vector<PairItem>& items, const char* env) {
     items.swap(out);
 }
 
+void order_leaf_items(std::vector<PairItem>& items, const char* env) {
+    if (items.size() < 2 * kXcds) return;
+    auto work = leaf_item_work;
+    if (!env || std::string(env) == "lpt") {
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const PairItem& a, const PairItem& b) { return work(a) > work(b); });
+        return;
+    }
+    if (env && std::string(env) == "tree") return;
+    xcd_interleave(items, work);
+}
+
 namespace {
 
 // Far-field items: largest first (their B rows are shared inside the
 // workgroup, so dispatch order only matters for the tail).
+// (The XCD-interleaved tree order measured 14% slower for far items.)
 void order_far_items(std::vector<FarItem>& items) {
     std::stable_sort(items.begin(), items.end(), [](const FarItem& a, const FarItem& b) {
         return int64_t(a.ktiles) * a.nrg > int64_t(b.ktiles) * b.nrg;
