@@ -61,8 +61,8 @@ inline size_t s_elem(int storage, int W) {
 }
 inline size_t p_elem(int storage) { return storage == HMX_F64 ? 8 : 4; }
 
-template <typename T>
-T* upload(const std::vector<T>& v) {
+template <typename T, typename A>
+T* upload(const std::vector<T, A>& v) {
     T* d = nullptr;
     if (v.empty()) return nullptr;
     HIPCHECK(hipMalloc(&d, v.size() * sizeof(T)));

@@ -11,6 +11,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/hmx.h"
@@ -104,6 +106,30 @@ double sumsq_lanes(const double* x, int64_t n);
 double dot_lanes64(const double* x, const double* y, int64_t n);
 
 // ---- H-matrix -----------------------------------------------------------------
+// Allocator whose construct() without arguments default-initialises: resizing a
+// vector of doubles then leaves the new elements unset instead of writing zeros.
+// The H-matrix payloads (tens of GB) are always written in full afterwards, by
+// several threads, so a single-threaded zero fill first would only cost time.
+template <typename T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <typename U>
+    DefaultInitAlloc(const DefaultInitAlloc<U>&) noexcept {}
+    template <typename U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+using Payload = std::vector<double, DefaultInitAlloc<double>>;
+
 struct DenseBlock {
     int64_t r0, r1, c0, c1;  // 1-based inclusive (hmatrix.jl:185)
     int64_t off;             // 0-based offset in the flattened dense payload
@@ -127,8 +153,8 @@ struct HMatrix {
     std::vector<int64_t> tmap, smap;  // 1-based
     std::vector<DenseBlock> dense;    // direct list, then storage fallbacks
     std::vector<LowRankBlock> lowrank;
-    std::vector<double> U, V;         // flattened row-major payloads (FP64)
-    std::vector<double> D;            // flattened dense payload; filled only for
+    Payload U, V;                     // flattened row-major payloads (FP64)
+    Payload D;                        // flattened dense payload; filled only for
                                       // callback kernels or on export
     bool D_ready = false;
     int64_t R = 0, dlen = 0;

@@ -122,7 +122,7 @@ void EntrySource::block(const int64_t* rows, int64_t nr, const int64_t* cols, in
 // ---- H-matrix ----------------------------------------------------------------
 void HMatrix::ensure_dense_payload() {
     if (D_ready) return;
-    D.assign(static_cast<size_t>(dlen), 0.0);
+    D.resize(static_cast<size_t>(dlen));  // every element is written below
     const int threads = src.thread_safe() ? default_threads() : 1;
     parallel_for(static_cast<int64_t>(dense.size()), threads, [&](int64_t b) {
         const DenseBlock& d = dense[b];
