@@ -86,3 +86,23 @@ def test_split_batch_equals_members(cuda, monkeypatch, storage, dtype, nrhs):
         assert torch.equal(Ys[i], y), "member %d differs from its own product" % i
         Yo = np.stack([O.mul_layout(layout_ns(h), Xn[i][:, c]) for c in range(nrhs)], axis=1)
         assert rel(y.cpu().numpy(), Yo) <= (1e-12 if storage == "f64" else 1e-5)
+
+
+def test_dispatch_order_changes_nothing(cuda, monkeypatch):
+    """Each leaf item's arithmetic is independent of when it runs, so every
+    dispatch order of the leaf items (HMX_ITEM_ORDER) gives the same bits."""
+    import torch
+    import hmatrixgpu as H
+    n = 6000
+    pts = uniform_points(n, 75)
+    T = H.ClusterTree(pts, max_points_per_leaf=32)
+    hm = H.HMatrix(H.LaplaceKernel(pts), T, T, eta=1.5, eps=1e-6, storage=())
+    X = np.random.default_rng(76).standard_normal((n, 16))
+    _force(monkeypatch)
+    ys = {}
+    for order in ("xcd", "lpt", "tree"):
+        monkeypatch.setenv("HMX_ITEM_ORDER", order)
+        hm.materialize("f32")
+        ys[order] = _apply(torch, H, hm, X, "f32", torch.float32, cuda)
+        hm.release("f32")
+    assert np.array_equal(ys["xcd"], ys["lpt"]) and np.array_equal(ys["xcd"], ys["tree"])
