@@ -577,6 +577,9 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<
     Staging stage[2];
     int next_stage = 0;
     const bool verbose = std::getenv("HMX_VERBOSE") != nullptr;
+    auto wall = [&] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    };
     auto wait_jobs = [&] {
         for (auto& st : stage)
             if (st.job.valid()) st.job.get();
@@ -870,10 +873,10 @@ void gpu_aca(const HMatrix& h, const std::vector<AcaBlock>& blocks, std::vector<
                 pass++;
                 if (verbose)
                     std::fprintf(stderr, "[hmx] GPU ACA+ chunk %lld pass %d: %lld of %lld blocks done, "
-                                         "%.1f ms kernels\n",
+                                         "%.1f ms ACA+ kernels, %.1f ms recompression, %.2f s wall\n",
                                  static_cast<long long>(S.chunks), pass,
                                  static_cast<long long>(done_blocks), static_cast<long long>(nb),
-                                 S.kernel_ms);
+                                 S.kernel_ms, S.recompress_ms, wall());
             }
             S.passes = std::max<int64_t>(S.passes, pass);
             for (void* p : owned)
