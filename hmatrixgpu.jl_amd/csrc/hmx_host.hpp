@@ -129,6 +129,10 @@ struct DefaultInitAlloc : std::allocator<T> {
     }
 };
 using Payload = std::vector<double, DefaultInitAlloc<double>>;
+// Ask for transparent huge pages on a large, not yet touched buffer (its first
+// writes then fault 2 MiB pages instead of 4 KiB ones); a no-op where the
+// kernel does not honour the advice.
+void advise_huge_pages(void* p, size_t bytes);
 
 struct DenseBlock {
     int64_t r0, r1, c0, c1;  // 1-based inclusive (hmatrix.jl:185)

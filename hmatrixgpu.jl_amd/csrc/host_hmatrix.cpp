@@ -19,6 +19,8 @@
 
 #include "hmx_host.hpp"
 
+#include <sys/mman.h>
+
 namespace hmx {
 
 int default_threads() {
@@ -120,9 +122,17 @@ void EntrySource::block(const int64_t* rows, int64_t nr, const int64_t* cols, in
 }
 
 // ---- H-matrix ----------------------------------------------------------------
+void advise_huge_pages(void* p, size_t bytes) {
+    const uintptr_t huge = uintptr_t(2) << 20;
+    const uintptr_t a = (reinterpret_cast<uintptr_t>(p) + huge - 1) & ~(huge - 1);
+    const uintptr_t e = (reinterpret_cast<uintptr_t>(p) + bytes) & ~(huge - 1);
+    if (p && e > a) (void)madvise(reinterpret_cast<void*>(a), e - a, MADV_HUGEPAGE);
+}
+
 void HMatrix::ensure_dense_payload() {
     if (D_ready) return;
     D.resize(static_cast<size_t>(dlen));  // every element is written below
+    advise_huge_pages(D.data(), D.size() * sizeof(double));
     const int threads = src.thread_safe() ? default_threads() : 1;
     parallel_for(static_cast<int64_t>(dense.size()), threads, [&](int64_t b) {
         const DenseBlock& d = dense[b];
@@ -317,6 +327,8 @@ std::unique_ptr<HMatrix> build_hmatrix(const ClusterTree& T, const ClusterTree& 
     h->R = vx;
     h->U.resize(static_cast<size_t>(ulen));
     h->V.resize(static_cast<size_t>(vlen));
+    advise_huge_pages(h->U.data(), h->U.size() * sizeof(double));
+    advise_huge_pages(h->V.data(), h->V.size() * sizeof(double));
     parallel_for(static_cast<int64_t>(keep.size()), threads, [&](int64_t q) {
         LowRank& f = lr[keep[q]];
         const LowRankBlock& L = h->lowrank[q];

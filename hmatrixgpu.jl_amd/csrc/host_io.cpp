@@ -312,10 +312,13 @@ std::unique_ptr<HMatrix> load_hmatrix(const char* path) {
     }
     h->U.resize(hd.count[kU]);
     h->V.resize(hd.count[kV]);
+    advise_huge_pages(h->U.data(), h->U.size() * sizeof(double));
+    advise_huge_pages(h->V.data(), h->V.size() * sizeof(double));
     r.section(h->U.data(), h->U.size() * 8);
     r.section(h->V.data(), h->V.size() * 8);
     if (has_dense) {
         h->D.resize(hd.dlen);
+        advise_huge_pages(h->D.data(), h->D.size() * sizeof(double));
         r.section(h->D.data(), h->D.size() * 8);
     }
     if (has_coords) {
