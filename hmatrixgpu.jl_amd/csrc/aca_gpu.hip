@@ -379,44 +379,108 @@ __device__ __forceinline__ void rec_svd(RecTask& T, const double* QU, const doub
     // disjoint rotations commute exactly)
     const double tol = 2.220446049250313e-16;
     const int kp = k + (k & 1);
-    for (int sweep = 0; sweep < 80; sweep++) {
-        bool rotated = false;
-        for (int rnd = 0; rnd + 1 < kp; rnd++) {
-            for (int pi = lane; pi < kp / 2; pi += 64) {
-                const int a = pi == 0 ? 0 : 1 + (pi - 1 + rnd) % (kp - 1);
-                const int b = 1 + (kp - 2 - pi + rnd) % (kp - 1);
-                const int p = min(a, b), q = max(a, b);
-                if (q >= k) continue;
-                double* ap = M + p * k;
-                double* aq = M + q * k;
-                double al = 0.0, be = 0.0, ga = 0.0;
-                for (int i = 0; i < k; i++) {
-                    const double x = ap[i], y = aq[i];
-                    const double xx = x * x, yy = y * y, xy = x * y;
-                    al = al + xx;
-                    be = be + yy;
-                    ga = ga + xy;
+    if constexpr (IN_LDS) {
+        // k <= 32, so at most 16 pairs: the pair's lane forms the three sums in
+        // order (as the host does) and its rotation; four lanes per pair then
+        // apply it, each to every fourth element (elementwise, so the same bits)
+        __shared__ double rot_c[kLdsK / 2], rot_s[kLdsK / 2];
+        __shared__ int rot_pq[kLdsK / 2];
+        for (int sweep = 0; sweep < 80; sweep++) {
+            bool rotated = false;
+            for (int rnd = 0; rnd + 1 < kp; rnd++) {
+                if (lane < kp / 2) {
+                    const int pi = lane;
+                    const int a = pi == 0 ? 0 : 1 + (pi - 1 + rnd) % (kp - 1);
+                    const int b = 1 + (kp - 2 - pi + rnd) % (kp - 1);
+                    const int p = min(a, b), q = max(a, b);
+                    int code = -1;
+                    if (q < k) {
+                        const double* ap = M + p * k;
+                        const double* aq = M + q * k;
+                        double al = 0.0, be = 0.0, ga = 0.0;
+                        for (int i = 0; i < k; i++) {
+                            const double x = ap[i], y = aq[i];
+                            const double xx = x * x, yy = y * y, xy = x * y;
+                            al = al + xx;
+                            be = be + yy;
+                            ga = ga + xy;
+                        }
+                        if (!(ga == 0.0 || fabs(ga) <= tol * sqrt(al * be))) {
+                            rotated = true;
+                            const double zeta = (be - al) / (2.0 * ga);
+                            const double t =
+                                (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                            const double c = 1.0 / sqrt(1.0 + t * t);
+                            rot_c[pi] = c;
+                            rot_s[pi] = c * t;
+                            code = p | (q << 8);
+                        }
+                    }
+                    rot_pq[pi] = code;
                 }
-                if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
-                rotated = true;
-                const double zeta = (be - al) / (2.0 * ga);
-                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double c = 1.0 / sqrt(1.0 + t * t);
-                const double s = c * t;
-                double* zp = Z + p * k;
-                double* zq = Z + q * k;
-                for (int i = 0; i < k; i++) {
-                    const double x = ap[i], y = aq[i];
-                    ap[i] = c * x - s * y;
-                    aq[i] = s * x + c * y;
-                    const double zx = zp[i], zy = zq[i];
-                    zp[i] = c * zx - s * zy;
-                    zq[i] = s * zx + c * zy;
+                __syncthreads();
+                const int pi = lane >> 2, part = lane & 3;
+                if (pi < kp / 2 && rot_pq[pi] >= 0) {
+                    const int p = rot_pq[pi] & 255, q = rot_pq[pi] >> 8;
+                    const double c = rot_c[pi], s = rot_s[pi];
+                    double* ap = M + p * k;
+                    double* aq = M + q * k;
+                    double* zp = Z + p * k;
+                    double* zq = Z + q * k;
+                    for (int i = part; i < k; i += 4) {
+                        const double x = ap[i], y = aq[i];
+                        ap[i] = c * x - s * y;
+                        aq[i] = s * x + c * y;
+                        const double zx = zp[i], zy = zq[i];
+                        zp[i] = c * zx - s * zy;
+                        zq[i] = s * zx + c * zy;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
+            if (!__any(rotated)) break;
         }
-        if (!__any(rotated)) break;
+    } else {
+        for (int sweep = 0; sweep < 80; sweep++) {
+            bool rotated = false;
+            for (int rnd = 0; rnd + 1 < kp; rnd++) {
+                for (int pi = lane; pi < kp / 2; pi += 64) {
+                    const int a = pi == 0 ? 0 : 1 + (pi - 1 + rnd) % (kp - 1);
+                    const int b = 1 + (kp - 2 - pi + rnd) % (kp - 1);
+                    const int p = min(a, b), q = max(a, b);
+                    if (q >= k) continue;
+                    double* ap = M + p * k;
+                    double* aq = M + q * k;
+                    double al = 0.0, be = 0.0, ga = 0.0;
+                    for (int i = 0; i < k; i++) {
+                        const double x = ap[i], y = aq[i];
+                        const double xx = x * x, yy = y * y, xy = x * y;
+                        al = al + xx;
+                        be = be + yy;
+                        ga = ga + xy;
+                    }
+                    if (ga == 0.0 || fabs(ga) <= tol * sqrt(al * be)) continue;
+                    rotated = true;
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t =
+                        (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double c = 1.0 / sqrt(1.0 + t * t);
+                    const double s = c * t;
+                    double* zp = Z + p * k;
+                    double* zq = Z + q * k;
+                    for (int i = 0; i < k; i++) {
+                        const double x = ap[i], y = aq[i];
+                        ap[i] = c * x - s * y;
+                        aq[i] = s * x + c * y;
+                        const double zx = zp[i], zy = zq[i];
+                        zp[i] = c * zx - s * zy;
+                        zq[i] = s * zx + c * zy;
+                    }
+                }
+                __syncthreads();
+            }
+            if (!__any(rotated)) break;
+        }
     }
     for (int j = lane; j < k; j += 64) {
         double s = 0.0;
