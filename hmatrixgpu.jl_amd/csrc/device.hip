@@ -28,7 +28,8 @@
 //     a single RHS is a VALU GEMV over the same tiles.
 //
 // Stages per call (one launch each, stream ordered): gather X -> S, far-field
-// panels (V*X), partial reduce, leaf panels (near field + U*T) -> scatter to Y.
+// panels (V*X), partial reduce, leaf panels (near field + U*T) -> permuted
+// rows or piece partials, piece sum, unpermute -> Y.
 #include "device_op.hpp"
 
 #include <algorithm>

@@ -485,7 +485,8 @@ __device__ __forceinline__ const InstDesc& inst_of(const InstDesc& one, const In
 // (1 and 3) walk the same shared k-tiles at the same time, so each shared B
 // row is fetched once per workgroup and served to the second wave from L1.  A
 // single row group is split over all four waves.  Partial sums are reduced
-// through LDS and scattered to Y through target_map.
+// through LDS and written as whole rows, in permuted target order, to Yp (a
+// whole item) or to the leaf-piece partials (a k-range piece of an item).
 template <typename TS, int W, int CP, bool BATCH>
 __global__ __launch_bounds__(kWaves * 64) void k_leaf(const PairItem* __restrict__ items,
                                                       const InstDesc one,
