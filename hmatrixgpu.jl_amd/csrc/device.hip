@@ -33,6 +33,7 @@
 #include "device_op.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -225,6 +226,10 @@ void device_build(HMatrix& h, int storage) {
     if (h.src.kernel_id == HMX_KERNEL_DIPOLE && (h.target_dims != 3 || h.source_dims != 3))
         throw Error(HMX_ERR_UNSUPPORTED, "the dipole tensor kernel needs dims == 3");
     DeviceGuard guard(h.device);
+    const auto tb0 = std::chrono::steady_clock::now();
+    auto since = [&] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - tb0).count();
+    };
     auto op = std::make_unique<DeviceOp>();
     op->storage = storage;
     op->device = h.device;
@@ -408,6 +413,16 @@ void device_build(HMatrix& h, int storage) {
     std::vector<Item> gensB;  // one generation item per panel (col_off into cdesc)
     std::vector<int32_t> colidx;
     std::vector<ColDesc> cdesc;
+    {
+        // upper bounds (every unit's columns, padding of three segments per
+        // group), so that the long appends below never reallocate
+        int64_t cols = 0;
+        for (const Unit& u : units) cols += ncols_of(dense_of[u.leaf], lr_of[u.leaf]);
+        const int64_t pad = 3 * TK * static_cast<int64_t>(groups_b.size());
+        colidx.reserve(static_cast<size_t>(cols + pad));
+        cdesc.reserve(static_cast<size_t>(cols + pad));
+        gensB.reserve(3 * groups_b.size());
+    }
     // append the columns of the listed blocks for row group starting at r0
     auto add_colidx = [&](const std::vector<int32_t>& dl, const std::vector<int32_t>& ll) {
         const int64_t c0 = static_cast<int64_t>(colidx.size());
@@ -493,6 +508,7 @@ void device_build(HMatrix& h, int storage) {
         itemsB.push_back(it);
     }
     op->panel_elems = data_off;
+    const double t_plan = since();
     op->npartial = npartial;
 
     // -- upload and generate ---------------------------------------------------------
@@ -586,6 +602,9 @@ void device_build(HMatrix& h, int storage) {
     op->nG = static_cast<int64_t>(groups.size());
     op->lgroups = upload(lgroups);
     op->nLG = static_cast<int64_t>(lgroups.size());
+    if (std::getenv("HMX_VERBOSE"))
+        std::fprintf(stderr, "[hmx] storage %d device build: plan %.2f s, total %.2f s\n", storage,
+                     t_plan, since());
     h.dev[storage].reset(op.release());
 }
 
